@@ -1,0 +1,236 @@
+"""Benchmark: fwd+bwd steps/s of the Gaussian rasterisation path on one MI355X.
+
+Workload (default ``full3m``): 3,000,000 splats (SH degree 3, anisotropic
+means + 10% far shell), one 1237x822 camera, rendering rgb + depth + normal,
+L1 losses against seeded random targets, backward into every Gaussian
+parameter and into the camera's 6-DoF pose (quaternion + translation) and
+intrinsics (fx, fy, cx, cy).  One step = one pass of MsplatRender.render_iter
++ loss + backward over that input, inputs resident in HBM.
+
+Prints one JSON line (see the keys below), plus per-kernel times measured
+with device events on the launch stream, the HBM-roofline fraction of the
+dominant kernel, and the FP64 PyTorch-CPU oracle timed on a bounded sample on
+this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def quat_to_rotmat(q):
+    from pointrix_amd.renderer import quat_to_rotmat as q2r
+    return q2r(torch.nn.functional.normalize(q, dim=-1)[None])[0]
+
+
+def rotmat_to_quat(R):
+    """wxyz unit quaternion of a rotation matrix (Shepperd's method)."""
+    R = R.double()
+    tr = R.trace()
+    if tr > 0:
+        s = torch.sqrt(tr + 1.0) * 2
+        q = [0.25 * s, (R[2, 1] - R[1, 2]) / s, (R[0, 2] - R[2, 0]) / s, (R[1, 0] - R[0, 1]) / s]
+    elif R[0, 0] > R[1, 1] and R[0, 0] > R[2, 2]:
+        s = torch.sqrt(1.0 + R[0, 0] - R[1, 1] - R[2, 2]) * 2
+        q = [(R[2, 1] - R[1, 2]) / s, 0.25 * s, (R[0, 1] + R[1, 0]) / s, (R[0, 2] + R[2, 0]) / s]
+    elif R[1, 1] > R[2, 2]:
+        s = torch.sqrt(1.0 + R[1, 1] - R[0, 0] - R[2, 2]) * 2
+        q = [(R[0, 2] - R[2, 0]) / s, (R[0, 1] + R[1, 0]) / s, 0.25 * s, (R[1, 2] + R[2, 1]) / s]
+    else:
+        s = torch.sqrt(1.0 + R[2, 2] - R[0, 0] - R[1, 1]) * 2
+        q = [(R[1, 0] - R[0, 1]) / s, (R[0, 2] + R[2, 0]) / s, (R[1, 2] + R[2, 1]) / s, 0.25 * s]
+    return torch.stack([torch.as_tensor(v) for v in q]).float()
+
+
+class TrainableCamera:
+    """Camera pose as (wxyz quaternion, translation) + intrinsics, all leaves,
+    composed into the [4,4] extrinsic the way pointrix's CameraModel does
+    (pointrix/model/camera/camera_model.py:111-175)."""
+
+    def __init__(self, cam, dev):
+        self.width, self.height = cam.width, cam.height
+        self.q = rotmat_to_quat(cam.extrinsic_matrix[:3, :3]).to(dev).requires_grad_(True)
+        self.t = cam.extrinsic_matrix[:3, 3].clone().to(dev).requires_grad_(True)
+        self.intr = cam.intrinsic_params.clone().to(dev).requires_grad_(True)
+
+    def params(self):
+        return [self.q, self.t, self.intr]
+
+    def render_kwargs(self):
+        R = quat_to_rotmat(self.q)
+        Rt = torch.cat([R, self.t[:, None]], 1)
+        E = torch.cat([Rt, torch.tensor([[0.0, 0.0, 0.0, 1.0]], device=Rt.device)], 0)
+        center = -(R.T @ self.t)
+        return {"height": self.height, "width": self.width, "extrinsic_matrix": E,
+                "intrinsic_params": self.intr, "camera_center": center}
+
+
+def l1(a, b):
+    return (a - b).abs().mean()
+
+
+def make_step(renderer, scene, cam, targets):
+    params = scene.params() + cam.params()
+
+    def step(stats=None):
+        inp = scene.render_inputs()
+        res = renderer.render_iter(**cam.render_kwargs(), **inp)
+        f = res["rendered_features_split"]
+        loss = l1(f["rgb"], targets["rgb"])
+        if "depth" in f:
+            loss = loss + 0.1 * l1(f["depth"], targets["depth"])
+        if "normal" in f:
+            loss = loss + 0.1 * l1(f["normal"], targets["normal"])
+        loss.backward()
+        if stats is not None:
+            stats["visible"] = int(res["visibility"].sum().item())
+        for p in params:
+            p.grad = None
+        return res
+
+    return step
+
+
+def algorithmic_bytes(N, K, C, P, T, Nv, M):
+    """Compulsory HBM bytes per stage: every stage reads its inputs once and
+    writes its outputs once; the sort is one read + one write of key + value."""
+    return {
+        "preprocess_fwd": N * (4 * (11 + 3 * K) + 44),
+        "binning": 8 * N + 20 * Nv + 44 * M + 8 * T,
+        "alpha_blending_fwd": 8 * T + 4 * M + 4 * Nv * (6 + C) + 4 * P * (C + 2),
+        "alpha_blending_bwd": 4 * P * (C + 2) + 8 * T + 4 * M + 4 * Nv * (6 + C) + 4 * Nv * (8 + C),
+        "preprocess_bwd": N * (8 * (11 + 3 * K) + 80),
+    }
+
+
+def cpu_baseline(workload_fn, div, seed=0):
+    """FP64 oracle fwd+bwd on a random 1/div subset of the splats (same camera),
+    scaled to steps/s of the full scene by the subset factor."""
+    from oracle import torch64 as o64
+    scene, cams = workload_fn(seed)
+    cam = cams[0]
+    n = len(scene)
+    g = torch.Generator().manual_seed(seed + 7)
+    idx = torch.randperm(n, generator=g)[: max(1, n // div)]
+    inp = {k: v[idx].double().requires_grad_(True)
+           for k, v in scene.render_inputs().items()}
+    threads = torch.get_num_threads()
+    t0 = time.perf_counter()
+    ref = o64.render_iter(cam.height, cam.width, cam.extrinsic_matrix.double(),
+                          cam.intrinsic_params.double(), cam.camera_center.double(),
+                          inp["position"], inp["opacity"], inp["scaling"], inp["rotation"],
+                          inp["shs"], sh_degree=3, render_depth=True)
+    loss = sum(v.abs().mean() for v in ref["rendered_features_split"].values())
+    loss.backward()
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / (dt * div), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 PyTorch-CPU oracle, fwd+bwd (rgb+depth) on a random 1/{div} subset "
+                      f"({len(idx)} splats) of the scene, same camera; {dt:.1f} s measured, "
+                      f"scaled x{div}; os.cpu_count()={os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="full3m")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-div", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pointrix_amd import ops
+    from pointrix_amd import scene as S
+    from pointrix_amd.renderer import MsplatRender
+
+    dev = torch.device("cuda:0")
+    wl = S.WORKLOADS[args.workload]
+    scene_cpu, cams = wl(0)
+    scene = scene_cpu.to(dev, requires_grad=True)
+    cam = TrainableCamera(cams[0], dev)
+    W, H = cam.width, cam.height
+    g = torch.Generator().manual_seed(123)
+    targets = {"rgb": torch.rand(3, H, W, generator=g).to(dev),
+               "depth": (torch.rand(1, H, W, generator=g) * 8).to(dev),
+               "normal": torch.rand(3, H, W, generator=g).to(dev)}
+    renderer = MsplatRender({"render_depth": True, "render_normal": True, "max_sh_degree": 3},
+                            white_bg=False, device="cuda")
+    renderer.sh_degree = 3
+    step = make_step(renderer, scene, cam, targets)
+    print(f"[bench] workload {args.workload}: N={len(scene)} {W}x{H}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    ops.timer = ops.KernelTimer()
+    stats = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(stats if i == args.steps - 1 else None)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = ops.timer.summary()
+    ops.timer = None
+    ms = dt * 1000 / args.steps
+    print("[bench] timed steps done", file=sys.stderr, flush=True)
+
+    # sizes for the algorithmic-byte model
+    with torch.no_grad():
+        res = renderer.render_iter(**cam.render_kwargs(), **scene.render_inputs())
+    Nv = int(res["visibility"].sum().item())
+    M = ops.last_instance_count
+    N = len(scene)
+    K, C, P = 16, 7, W * H
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    ab = algorithmic_bytes(N, K, C, P, T, Nv, M)
+    per_kernel = {k: {"launches": n, "avg_ms": tot / max(n, 1)} for k, (n, tot) in kt.items()}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches"])
+    dom_bytes = ab.get(dom, ab["binning"] if dom.startswith("sort") else None)
+    achieved = dom_bytes / (per_kernel[dom]["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None}
+    print(f"[bench] per-kernel: {json.dumps(per_kernel)}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wl, args.cpu_div)
+
+    line = {
+        "metric": "fwd+bwd steps/s @3M Gaussians 1237x822",
+        "value": round(1000.0 / ms, 3), "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random scene and targets)",
+        "config": {"workload": f"{args.workload}: {N} splats SH3, {W}x{H}, rgb+depth+normal, "
+                               "grads to splats + camera pose/intrinsics",
+                   "splats": N, "width": W, "height": H, "visible": Nv, "instances": M,
+                   "mpixel_per_s": round(P / 1e6 * 1000.0 / ms, 2),
+                   "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

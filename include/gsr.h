@@ -1,0 +1,122 @@
+/*
+ * gsr.h -- C ABI of libgsr, the gfx950 (MI355X) differentiable Gaussian
+ * rasteriser.
+ *
+ * Every entry point replaces one msplat call made by the reference renderer,
+ * pointrix/model/renderer/msplat.py (cited per function), forward and
+ * backward.  All pointers are device pointers unless noted; tensors are dense
+ * row-major fp32 / int32 / uint8 arrays owned by the caller.  Work is enqueued
+ * on `stream` (a hipStream_t) and never synchronises the host; the calls are
+ * capturable into a HIP graph.  Return value: 0 on success, a negative
+ * hipError_t-derived code on a launch failure, GSR_ERR_ARG (-1000) on a bad
+ * argument; gsr_last_error() describes the last failure of the calling thread.
+ *
+ * Conventions (fixed by this library, documented in DESIGN.md): intrinsics
+ * intr = (fx, fy, cx, cy); extrinsics extr = 3x4 row-major world-to-camera
+ * [R|t]; pixel centres at integer coordinates; 16x16 tiles unless a tile shape
+ * argument says otherwise; cov3d packed (xx, xy, xz, yy, yz, zz); quaternions
+ * wxyz.
+ */
+#ifndef GSR_H_
+#define GSR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int gsr_version(void);
+const char *gsr_last_error(void);
+
+/* ---- msplat.project_point (msplat.py:109-113) ---------------------------
+ * uv[N,2], depth[N] (depth 0 and uv 0 for culled points: z <= nearest or
+ * |2u/W - 1| > extent or |2v/H - 1| > extent). */
+int gsr_project_point_fwd(const float *xyz, int64_t n, const float *intr, const float *extr,
+                          int W, int H, float nearest, float extent, float *uv, float *depth,
+                          void *stream);
+/* Backward.  depth is the forward output (0 marks culled points).  d_intr[4]
+ * and d_extr[12] are accumulated (+=) and may be NULL. */
+int gsr_project_point_bwd(const float *xyz, int64_t n, const float *intr, const float *extr,
+                          const float *depth, const float *d_uv, const float *d_depth,
+                          float *d_xyz, float *d_intr, float *d_extr, void *stream);
+
+/* ---- msplat.compute_cov3d (msplat.py:118) -------------------------------
+ * cov3d[N,6] = R(q) diag(s)^2 R(q)^T packed; 0 where visible[i] == 0. */
+int gsr_compute_cov3d_fwd(const float *scale, const float *rot, const uint8_t *visible,
+                          int64_t n, float *cov3d, void *stream);
+int gsr_compute_cov3d_bwd(const float *scale, const float *rot, const uint8_t *visible,
+                          int64_t n, const float *d_cov3d, float *d_scale, float *d_rot,
+                          void *stream);
+
+/* ---- msplat.ewa_project (msplat.py:121-130) ------------------------------
+ * conic[N,3] (inverse EWA 2D covariance), radius[N] (3-sigma, pixels) and
+ * tiles[N] (tiles touched; 16x16 tiles).  All 0 for invisible points. */
+int gsr_ewa_project_fwd(const float *xyz, const float *cov3d, int64_t n, const float *intr,
+                        const float *extr, const float *uv, int W, int H,
+                        const uint8_t *visible, float *conic, int32_t *radius,
+                        int32_t *tiles, void *stream);
+/* Backward.  radius is the forward output (0 marks skipped points).  d_intr
+ * and d_extr are accumulated and may be NULL. */
+int gsr_ewa_project_bwd(const float *xyz, const float *cov3d, int64_t n, const float *intr,
+                        const float *extr, int W, int H, const int32_t *radius,
+                        const float *d_conic, float *d_xyz, float *d_cov3d, float *d_intr,
+                        float *d_extr, void *stream);
+
+/* ---- msplat.compute_sh (msplat.py:104) -----------------------------------
+ * out[n,c] = sum_{k < K'} shs[n,c,k] * Y_k(dirs[n]) with K' = min(K,
+ * (degree+1)^2) (degree < 0: K' = K).  shs is addressed through element
+ * strides (s_n, s_c, s_k) so the permuted [N,16,3] -> [N,3,16] view the
+ * reference passes needs no copy. */
+int gsr_compute_sh_fwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
+                       const float *dirs, int64_t n, int C, int K, int degree, float *out,
+                       void *stream);
+/* d_shs is dense [N,C,K] (overwritten, zero for k >= K'); d_dirs[N,3]
+ * overwritten, may be NULL. */
+int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
+                       const float *dirs, int64_t n, int C, int K, int degree,
+                       const float *d_out, float *d_shs, float *d_dirs, void *stream);
+
+/* ---- msplat.sort_gaussian (msplat.py:133-135) ----------------------------
+ * Two phases around one host read of M (the instance count):
+ *   gsr_sort_count: per-splat tile counts from (uv, radius), depth-sort of
+ *     the splats that touch a tile, exclusive scan -> *m_out (device int32).
+ *   gsr_sort_emit: one (tile, id) pair per touched tile, stable tile sort,
+ *     tile ranges.  ids_out[M] is ordered by (tile, f32 depth bits, splat id);
+ *     tile_range[T,2] = [first, last+1), (0, 0) for empty tiles.
+ * ws_gauss must hold gsr_sort_ws_gauss_bytes(n) bytes; ws_inst
+ * gsr_sort_ws_inst_bytes(M) bytes; both 256-byte aligned. */
+size_t gsr_sort_ws_gauss_bytes(int64_t n);
+size_t gsr_sort_ws_inst_bytes(int64_t m);
+int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                   int W, int H, int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
+                   void *stream);
+int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
+                  int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                  int32_t *tile_range, void *stream);
+
+/* ---- msplat.alpha_blending (msplat.py:148-151) ---------------------------
+ * out[C,H,W] = sum_i f_i alpha_i T_i + bg * T_final, front to back over the
+ * tile's depth-sorted splats.  Also writes T_final[H,W] and
+ * n_contrib[H,W] (1 + index within the tile range of the last contributing
+ * splat), which the backward pass needs.  C in 1..16. */
+int gsr_alpha_blending_fwd(const float *uv, const float *conic, const float *opacity,
+                           const float *feature, int C, const int32_t *ids,
+                           const int32_t *tile_range, float bg, int W, int H, int tile_w,
+                           int tile_h, float *out, float *T_final, int32_t *n_contrib,
+                           void *stream);
+/* Backward: d_out[C,H,W], d_T_final[H,W] (may be NULL) -> gradients,
+ * accumulated (+=) into d_uv[N,2], d_conic[N,3], d_opacity[N], d_feature[N,C]
+ * (caller zero-fills). */
+int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opacity,
+                           const float *feature, int C, const int32_t *ids,
+                           const int32_t *tile_range, float bg, int W, int H, int tile_w,
+                           int tile_h, const float *T_final, const int32_t *n_contrib,
+                           const float *d_out, const float *d_T_final, float *d_uv,
+                           float *d_conic, float *d_opacity, float *d_feature, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSR_H_ */

@@ -1,0 +1,323 @@
+/*
+ * ref32.c -- FP32 C restatement of the forward rasterisation path (test
+ * infrastructure; built by oracle/build_ref32.py, loaded by oracle/ref32.py).
+ *
+ * Restates, in plain sequential C, the forward of the six msplat ops the
+ * reference renderer calls (pointrix/model/renderer/msplat.py:104-151) under
+ * the conventions of oracle/torch64.py, with every float operation in the
+ * order the HIP kernels perform it (compiled with -ffp-contract=off: one
+ * rounding per written operator, fmaf only where written).  The HIP results
+ * must equal these bit for bit: radius, tiles touched, sorted ids, tile
+ * ranges and per-pixel contributor counts (the integer outputs), and also the
+ * float outputs.  The exp used by the blend is defined here from its method
+ * (Cody-Waite reduction + degree-7 Taylor + exact 2^k scaling), not copied.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TILE_DEFAULT 16
+
+static float ln2_hi(void) { return 0.693145751953125f; } /* 15-bit head of ln 2 */
+static float ln2_lo(void) { return (float)(log(2.0) - 0.693145751953125); }
+
+static float exp_neg(float x) {
+  const float log2e = (float)(1.0 / log(2.0));
+  float kf, r, p;
+  x = fmaxf(x, -80.0f);
+  kf = rintf(x * log2e);
+  r = fmaf(kf, -ln2_hi(), x);
+  r = fmaf(kf, -ln2_lo(), r);
+  p = (float)(1.0 / 5040.0);
+  p = fmaf(p, r, (float)(1.0 / 720.0));
+  p = fmaf(p, r, (float)(1.0 / 120.0));
+  p = fmaf(p, r, (float)(1.0 / 24.0));
+  p = fmaf(p, r, (float)(1.0 / 6.0));
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)kf);
+}
+
+float ref_exp_neg(float x) { return exp_neg(x); }
+
+/* camera: intr = fx fy cx cy; extr 3x4 row-major */
+static void cam_point(const float *E, const float *x, float *p) {
+  p[0] = E[0] * x[0] + E[1] * x[1] + E[2] * x[2] + E[3];
+  p[1] = E[4] * x[0] + E[5] * x[1] + E[6] * x[2] + E[7];
+  p[2] = E[8] * x[0] + E[9] * x[1] + E[10] * x[2] + E[11];
+}
+
+void ref_project(const float *xyz, int64_t n, const float *K, const float *E, int W, int H,
+                 float nearest, float extent, float *uv, float *depth) {
+  for (int64_t i = 0; i < n; ++i) {
+    float p[3], u, v, lo_x, hi_x, lo_y, hi_y;
+    cam_point(E, xyz + 3 * i, p);
+    uv[2 * i] = 0.0f;
+    uv[2 * i + 1] = 0.0f;
+    depth[i] = 0.0f;
+    if (!(p[2] > nearest)) continue;
+    u = K[0] * p[0] / p[2] + K[2];
+    v = K[1] * p[1] / p[2] + K[3];
+    lo_x = 0.5f * (float)W * (1.0f - extent);
+    hi_x = 0.5f * (float)W * (1.0f + extent);
+    lo_y = 0.5f * (float)H * (1.0f - extent);
+    hi_y = 0.5f * (float)H * (1.0f + extent);
+    if (u < lo_x || u > hi_x || v < lo_y || v > hi_y) continue;
+    uv[2 * i] = u;
+    uv[2 * i + 1] = v;
+    depth[i] = p[2];
+  }
+}
+
+static void rotmat(const float *q, float *R) {
+  const float w = q[0], x = q[1], y = q[2], z = q[3];
+  const float x2 = x * x, y2 = y * y, z2 = z * z, w2 = w * w;
+  const float xy = x * y, zw = z * w, xz = x * z, yw = y * w, yz = y * z, xw = x * w;
+  R[0] = x2 - y2 - z2 + w2;
+  R[1] = 2.0f * (xy - zw);
+  R[2] = 2.0f * (xz + yw);
+  R[3] = 2.0f * (xy + zw);
+  R[4] = -x2 + y2 - z2 + w2;
+  R[5] = 2.0f * (yz - xw);
+  R[6] = 2.0f * (xz - yw);
+  R[7] = 2.0f * (yz + xw);
+  R[8] = -x2 - y2 + z2 + w2;
+}
+
+void ref_cov3d(const float *scale, const float *rot, const uint8_t *vis, int64_t n, float *cov) {
+  static const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+  for (int64_t i = 0; i < n; ++i) {
+    float R[9], M[9];
+    int k, r;
+    for (k = 0; k < 6; ++k) cov[6 * i + k] = 0.0f;
+    if (!vis[i]) continue;
+    rotmat(rot + 4 * i, R);
+    for (r = 0; r < 3; ++r) {
+      M[3 * r + 0] = R[3 * r + 0] * scale[3 * i + 0];
+      M[3 * r + 1] = R[3 * r + 1] * scale[3 * i + 1];
+      M[3 * r + 2] = R[3 * r + 2] * scale[3 * i + 2];
+    }
+    for (k = 0; k < 6; ++k) {
+      const int a = ii[k], b = jj[k];
+      cov[6 * i + k] = M[3 * a] * M[3 * b] + M[3 * a + 1] * M[3 * b + 1] + M[3 * a + 2] * M[3 * b + 2];
+    }
+  }
+}
+
+static void rect_of(float u, float v, int r, int gx, int gy, int tw, int th, int *x0, int *x1,
+                    int *y0, int *y1) {
+  const float rf = (float)r, ftw = (float)tw, fth = (float)th;
+  *x0 = (int)fminf(fmaxf((u - rf) / ftw, 0.0f), (float)gx);
+  *x1 = (int)fminf(fmaxf(((u + rf) + (ftw - 1.0f)) / ftw, 0.0f), (float)gx);
+  *y0 = (int)fminf(fmaxf((v - rf) / fth, 0.0f), (float)gy);
+  *y1 = (int)fminf(fmaxf(((v + rf) + (fth - 1.0f)) / fth, 0.0f), (float)gy);
+}
+
+void ref_ewa(const float *xyz, const float *cov, int64_t n, const float *K, const float *E,
+             const float *uv, int W, int H, const uint8_t *vis, float *conic, int32_t *radius,
+             int32_t *tiles) {
+  const int gx = (W + TILE_DEFAULT - 1) / TILE_DEFAULT, gy = (H + TILE_DEFAULT - 1) / TILE_DEFAULT;
+  const float lowpass = 0.3f, jclamp = 1.3f;
+  for (int64_t i = 0; i < n; ++i) {
+    float p[3], z, limx, limy, txz, tyz, tx, ty, J00, J02, J11, J12, T[6], U[6], Sf[9];
+    float a, b, c, det, inv, mid, disc, lam;
+    const float *S = cov + 6 * i;
+    int r, x0, x1, y0, y1, j, q;
+    conic[3 * i] = conic[3 * i + 1] = conic[3 * i + 2] = 0.0f;
+    radius[i] = 0;
+    tiles[i] = 0;
+    if (!vis[i]) continue;
+    cam_point(E, xyz + 3 * i, p);
+    z = p[2];
+    limx = jclamp * (0.5f * (float)W / K[0]);
+    limy = jclamp * (0.5f * (float)H / K[1]);
+    txz = fminf(fmaxf(p[0] / z, -limx), limx);
+    tyz = fminf(fmaxf(p[1] / z, -limy), limy);
+    tx = txz * z;
+    ty = tyz * z;
+    J00 = K[0] / z;
+    J02 = -(K[0] * tx) / (z * z);
+    J11 = K[1] / z;
+    J12 = -(K[1] * ty) / (z * z);
+    for (j = 0; j < 3; ++j) {
+      T[j] = J00 * E[j] + J02 * E[8 + j];
+      T[3 + j] = J11 * E[4 + j] + J12 * E[8 + j];
+    }
+    Sf[0] = S[0]; Sf[1] = S[1]; Sf[2] = S[2];
+    Sf[3] = S[1]; Sf[4] = S[3]; Sf[5] = S[4];
+    Sf[6] = S[2]; Sf[7] = S[4]; Sf[8] = S[5];
+    for (q = 0; q < 2; ++q)
+      for (j = 0; j < 3; ++j)
+        U[3 * q + j] = T[3 * q] * Sf[j] + T[3 * q + 1] * Sf[3 + j] + T[3 * q + 2] * Sf[6 + j];
+    a = U[0] * T[0] + U[1] * T[1] + U[2] * T[2];
+    b = U[0] * T[3] + U[1] * T[4] + U[2] * T[5];
+    c = U[3] * T[3] + U[4] * T[4] + U[5] * T[5];
+    a = a + lowpass;
+    c = c + lowpass;
+    det = a * c - b * b;
+    if (!(det > 0.0f)) continue;
+    inv = 1.0f / det;
+    mid = 0.5f * (a + c);
+    disc = fmaxf(mid * mid - det, 0.1f);
+    lam = mid + sqrtf(disc);
+    r = (int)ceilf(3.0f * sqrtf(lam));
+    rect_of(uv[2 * i], uv[2 * i + 1], r, gx, gy, TILE_DEFAULT, TILE_DEFAULT, &x0, &x1, &y0, &y1);
+    if ((x1 - x0) * (y1 - y0) <= 0) continue;
+    conic[3 * i] = c * inv;
+    conic[3 * i + 1] = -b * inv;
+    conic[3 * i + 2] = a * inv;
+    radius[i] = r;
+    tiles[i] = (x1 - x0) * (y1 - y0);
+  }
+}
+
+/* real SH, 3DGS sign convention; constants from their closed forms */
+void ref_sh(const float *shs, const float *dirs, int64_t n, int C, int K, int degree, float *out) {
+  const double pi = 3.14159265358979323846;
+  const float C0 = (float)(0.5 / sqrt(pi));
+  const float C1 = (float)sqrt(3.0 / (4.0 * pi));
+  const float C20 = (float)(0.5 * sqrt(15.0 / pi));
+  const float C22 = (float)(0.25 * sqrt(5.0 / pi));
+  const float C24 = (float)(0.25 * sqrt(15.0 / pi));
+  const float C30 = (float)(0.25 * sqrt(35.0 / (2.0 * pi)));
+  const float C31 = (float)(0.5 * sqrt(105.0 / pi));
+  const float C32 = (float)(0.25 * sqrt(21.0 / (2.0 * pi)));
+  const float C33 = (float)(0.25 * sqrt(7.0 / pi));
+  const float C35 = (float)(0.25 * sqrt(105.0 / pi));
+  int Ke = K;
+  if (degree >= 0 && (degree + 1) * (degree + 1) < K) Ke = (degree + 1) * (degree + 1);
+  for (int64_t i = 0; i < n; ++i) {
+    const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+    float Y[16];
+    int c, k;
+    Y[0] = C0;
+    if (Ke > 1) {
+      Y[1] = -C1 * y;
+      Y[2] = C1 * z;
+      Y[3] = -C1 * x;
+    }
+    if (Ke > 4) {
+      const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+      Y[4] = C20 * xy;
+      Y[5] = -C20 * yz;
+      Y[6] = C22 * (2.0f * zz - xx - yy);
+      Y[7] = -C20 * xz;
+      Y[8] = C24 * (xx - yy);
+      if (Ke > 9) {
+        Y[9] = -C30 * y * (3.0f * xx - yy);
+        Y[10] = C31 * xy * z;
+        Y[11] = -C32 * y * (4.0f * zz - xx - yy);
+        Y[12] = C33 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+        Y[13] = -C32 * x * (4.0f * zz - xx - yy);
+        Y[14] = C35 * z * (xx - yy);
+        Y[15] = -C30 * x * (xx - 3.0f * yy);
+      }
+    }
+    for (c = 0; c < C; ++c) {
+      const float *s = shs + (i * C + c) * K;
+      float acc = s[0] * Y[0];
+      for (k = 1; k < Ke; ++k) acc = acc + s[k] * Y[k];
+      out[i * C + c] = acc;
+    }
+  }
+}
+
+/* ---- binning ------------------------------------------------------------ */
+typedef struct {
+  uint64_t key;
+  uint32_t seq;
+  int32_t id;
+} Inst;
+
+static int inst_cmp(const void *pa, const void *pb) {
+  const Inst *a = (const Inst *)pa, *b = (const Inst *)pb;
+  if (a->key != b->key) return a->key < b->key ? -1 : 1;
+  return a->seq < b->seq ? -1 : (a->seq > b->seq ? 1 : 0);
+}
+
+int64_t ref_sort_count(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tw,
+                       int th) {
+  const int gx = (W + tw - 1) / tw, gy = (H + th - 1) / th;
+  int64_t m = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int x0, x1, y0, y1;
+    if (radius[i] <= 0) continue;
+    rect_of(uv[2 * i], uv[2 * i + 1], radius[i], gx, gy, tw, th, &x0, &x1, &y0, &y1);
+    m += (int64_t)(x1 - x0) * (y1 - y0);
+  }
+  return m;
+}
+
+/* ids[M] sorted by (tile, f32 depth bits, splat id); ranges[T][2]; returns M */
+int64_t ref_sort(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W,
+                 int H, int tw, int th, int32_t *ids, int32_t *ranges) {
+  const int gx = (W + tw - 1) / tw, gy = (H + th - 1) / th;
+  const int64_t m = ref_sort_count(uv, radius, n, W, H, tw, th);
+  Inst *v = (Inst *)malloc(sizeof(Inst) * (size_t)(m > 0 ? m : 1));
+  int64_t o = 0, j;
+  for (int64_t i = 0; i < n; ++i) {
+    int x0, x1, y0, y1, tx, ty;
+    uint32_t bits;
+    if (radius[i] <= 0) continue;
+    memcpy(&bits, depth + i, 4);
+    rect_of(uv[2 * i], uv[2 * i + 1], radius[i], gx, gy, tw, th, &x0, &x1, &y0, &y1);
+    for (ty = y0; ty < y1; ++ty)
+      for (tx = x0; tx < x1; ++tx) {
+        v[o].key = ((uint64_t)(uint32_t)(ty * gx + tx) << 32) | bits;
+        v[o].seq = (uint32_t)o;
+        v[o].id = (int32_t)i;
+        ++o;
+      }
+  }
+  qsort(v, (size_t)m, sizeof(Inst), inst_cmp);
+  memset(ranges, 0, sizeof(int32_t) * 2 * (size_t)gx * gy);
+  for (j = 0; j < m; ++j) {
+    const uint32_t t = (uint32_t)(v[j].key >> 32);
+    ids[j] = v[j].id;
+    if (j == 0 || (uint32_t)(v[j - 1].key >> 32) != t) ranges[2 * t] = (int32_t)j;
+    if (j == m - 1 || (uint32_t)(v[j + 1].key >> 32) != t) ranges[2 * t + 1] = (int32_t)(j + 1);
+  }
+  free(v);
+  return m;
+}
+
+/* ---- blending ----------------------------------------------------------- */
+void ref_blend(const float *uv, const float *conic, const float *opacity, const float *feat,
+               int C, const int32_t *ids, const int32_t *ranges, float bg, int W, int H, int tw,
+               int th, float *out, float *Tfin, int32_t *ncon) {
+  const int gx = (W + tw - 1) / tw;
+  const float power_cut = -5.6f, alpha_max = 0.99f, alpha_min = 1.0f / 255.0f, t_min = 1e-4f;
+  float acc[64];
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      const int t = (py / th) * gx + (px / tw);
+      const int s = ranges[2 * t], e = ranges[2 * t + 1];
+      const float fpx = (float)px, fpy = (float)py;
+      float T = 1.0f;
+      int last = 0, c, j;
+      for (c = 0; c < C; ++c) acc[c] = 0.0f;
+      for (j = s; j < e; ++j) {
+        const int g = ids[j];
+        const float ha = -0.5f * conic[3 * g], nb = -conic[3 * g + 1],
+                    hc = -0.5f * conic[3 * g + 2], o = opacity[g];
+        const float dx = uv[2 * g] - fpx, dy = uv[2 * g + 1] - fpy;
+        const float power = fmaf(ha * dx, dx, fmaf(nb * dx, dy, (hc * dy) * dy));
+        float alpha, test_T, w;
+        if (power > 0.0f) continue;
+        if (power < power_cut && o <= 1.0f) continue;
+        alpha = fminf(alpha_max, o * exp_neg(power));
+        if (alpha < alpha_min) continue;
+        test_T = T * (1.0f - alpha);
+        if (test_T < t_min) break;
+        w = alpha * T;
+        for (c = 0; c < C; ++c) acc[c] = fmaf(feat[(int64_t)g * C + c], w, acc[c]);
+        T = test_T;
+        last = j - s + 1;
+      }
+      for (c = 0; c < C; ++c) out[(int64_t)c * H * W + (int64_t)py * W + px] = acc[c] + bg * T;
+      Tfin[(int64_t)py * W + px] = T;
+      ncon[(int64_t)py * W + px] = last;
+    }
+}

@@ -1,0 +1,482 @@
+// Tile binning (msplat.sort_gaussian, pointrix/model/renderer/msplat.py:133-135).
+//
+// Contract: every splat with radius > 0 emits one instance per tile of its
+// rectangle; instances are ordered by the 64-bit key (tile << 32 | f32 bits of
+// depth), ties by splat index; tile_range[t] = [first, last + 1).
+//
+// Design: the low 32 key bits (depth) are equal for all instances of a splat,
+// so the LSD radix sort of the 64-bit key runs its four depth passes over the
+// Nv visible splats instead of the M instances (M ~ 10-20 x Nv), then emits
+// instances in depth order and finishes with ceil(log2(T)/8) stable passes on
+// the tile bits over M.  Result is identical to the full 64-bit sort.
+//
+// Radix passes: 8-bit digits, 4096 items per 256-thread block.  Ranking inside
+// a block uses wave64 ballots (match mask of lanes with equal digits) and
+// per-wave LDS counters, which keeps the sort stable with two barriers per
+// block.  Element counts live on the device so no pass needs a host sync.
+#include "gsr_internal.h"
+
+using namespace gsr;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTileItems = kThreads * kItems;  // 4096
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+__device__ __forceinline__ uint32_t count_of(const uint32_t *n_ptr, uint32_t cap) {
+  return n_ptr ? min(*n_ptr, cap) : cap;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
+  __shared__ uint32_t s_w[kThreads / 64];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) {
+    const uint32_t s = s_w[k];
+    if (k < w) base += s;
+    total += s;
+  }
+  __syncthreads();
+  return base + inc - v;
+}
+
+// --------------------------------------------------------------------------
+// Generic three-kernel exclusive scan (reduce / scan partials / downsweep).
+// --------------------------------------------------------------------------
+template <class Load>
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, const uint32_t *n_ptr,
+                                                               uint32_t cap,
+                                                               uint32_t *partials) {
+  const uint32_t n = count_of(n_ptr, cap);
+  const uint32_t base = blockIdx.x * kTileItems;
+  uint32_t s = 0;
+  if (base < n) {
+#pragma unroll 4
+    for (int k = 0; k < kItems; ++k) {
+      const uint32_t i = base + k * kThreads + threadIdx.x;
+      if (i < n) s += ld(i);
+    }
+  }
+  uint32_t total;
+  block_exclusive_scan(s, total);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One 1024-thread block scans the per-block partials in place (exclusive) and
+// writes the grand total.
+__global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t *partials, uint32_t nb,
+                                                             uint32_t *total_out,
+                                                             uint32_t *total_out2) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? partials[i] : 0u;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += t;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t wb = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t s = s_w[k];
+      if (k < w) wb += s;
+      tot += s;
+    }
+    const uint32_t carry = s_carry;
+    if (i < nb) partials[i] = carry + wb + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (total_out) *total_out = s_carry;
+    if (total_out2) *total_out2 = s_carry;
+  }
+}
+
+template <class Load, class Store>
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, Store st,
+                                                             const uint32_t *n_ptr, uint32_t cap,
+                                                             const uint32_t *partials) {
+  const uint32_t n = count_of(n_ptr, cap);
+  const uint32_t base = blockIdx.x * kTileItems;
+  if (base >= n) return;
+  const uint32_t first = base + threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = first + k;
+    v[k] = i < n ? ld(i) : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = partials[blockIdx.x] + block_exclusive_scan(s, total);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = first + k;
+    if (i < n) st(i, run, v[k]);
+    run += v[k];
+  }
+}
+
+struct LoadU32 {
+  const uint32_t *p;
+  __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
+};
+struct StoreU32 {
+  uint32_t *p;
+  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t) const { p[i] = excl; }
+};
+
+// --------------------------------------------------------------------------
+// Radix sort passes (32-bit key, 32-bit value, stable)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys,
+                                                              const uint32_t *n_ptr,
+                                                              uint32_t cap, int shift,
+                                                              uint32_t *__restrict__ hist,
+                                                              uint32_t nb) {
+  __shared__ uint32_t s_h[kRadix];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = count_of(n_ptr, cap);
+  const uint32_t base = blockIdx.x * kTileItems;
+  if (base < n) {
+#pragma unroll 4
+    for (int k = 0; k < kItems; ++k) {
+      const uint32_t i = base + k * kThreads + threadIdx.x;
+      if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
+    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
+    uint32_t cap, int shift, const uint32_t *__restrict__ hist_scanned, uint32_t nb) {
+  __shared__ uint32_t s_cnt[kThreads / 64][kRadix];
+  const uint32_t n = count_of(n_ptr, cap);
+  const uint32_t base = blockIdx.x * kTileItems;
+  if (base >= n) return;  // uniform over the block
+  const int w = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  uint32_t key[kItems];
+  int32_t val[kItems];
+  uint32_t rank[kItems];
+  const uint32_t wbase = base + w * (kTileItems / (kThreads / 64));
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = wbase + r * 64 + lane;
+    const bool valid = i < n;
+    key[r] = valid ? kin[i] : 0u;
+    val[r] = valid ? vin[i] : 0;
+    const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    uint32_t prior = 0;
+    if (valid) prior = s_cnt[w][d];
+    __builtin_amdgcn_wave_barrier();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    if (valid && lane == leader) s_cnt[w][d] = prior + (uint32_t)__popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    rank[r] = prior + (uint32_t)__popcll(m & lt);
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;
+    uint32_t run = hist_scanned[d * nb + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) {
+      const uint32_t t = s_cnt[k][d];
+      s_cnt[k][d] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = wbase + r * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+      const uint32_t pos = s_cnt[w][d] + rank[r];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Binning kernels
+// --------------------------------------------------------------------------
+struct Grid {
+  int gx, gy, tw, th;
+};
+
+__global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
+                                                         const int32_t *__restrict__ radius,
+                                                         uint32_t n, Grid g,
+                                                         uint32_t *__restrict__ cnt) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int r = radius[i];
+  uint32_t c = 0;
+  if (r > 0) {
+    const Rect q = tile_rect(uv[2 * i], uv[2 * i + 1], r, g.gx, g.gy, g.tw, g.th);
+    c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
+  }
+  cnt[i] = c;
+}
+
+struct LoadFlag {
+  const uint32_t *cnt;
+  __device__ uint32_t operator()(uint32_t i) const { return cnt[i] > 0 ? 1u : 0u; }
+};
+struct StoreCompact {
+  const float *depth;
+  uint32_t *keys;
+  int32_t *vals;
+  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t flag) const {
+    if (flag) {
+      keys[excl] = __float_as_uint(depth[i]);
+      vals[excl] = (int32_t)i;
+    }
+  }
+};
+struct LoadSortedCount {
+  const uint32_t *cnt;
+  const int32_t *ids;
+  __device__ uint32_t operator()(uint32_t s) const { return cnt[ids[s]]; }
+};
+
+__global__ __launch_bounds__(kThreads) void emit_kernel(
+    const float *__restrict__ uv, const int32_t *__restrict__ radius,
+    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
+    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t *__restrict__ tkeys,
+    int32_t *__restrict__ tvals) {
+  const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
+  if (s >= count_of(nv_ptr, cap)) return;
+  const int32_t id = sorted_ids[s];
+  const Rect q = tile_rect(uv[2 * id], uv[2 * id + 1], radius[id], g.gx, g.gy, g.tw, g.th);
+  uint32_t o = offsets[s];
+  for (int ty = q.y0; ty < q.y1; ++ty)
+    for (int tx = q.x0; tx < q.x1; ++tx) {
+      tkeys[o] = (uint32_t)(ty * g.gx + tx);
+      tvals[o] = id;
+      ++o;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void fill_kernel(int32_t *__restrict__ p, int64_t n,
+                                                        int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void ranges_kernel(const uint32_t *__restrict__ tkeys,
+                                                          const uint32_t *m_ptr, uint32_t cap,
+                                                          int32_t *__restrict__ ranges) {
+  const uint32_t m = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t M = count_of(m_ptr, cap);
+  if (m >= M) return;
+  const uint32_t t = tkeys[m];
+  if (m == 0 || tkeys[m - 1] != t) ranges[2 * t] = (int32_t)m;
+  if (m == M - 1 || tkeys[m + 1] != t) ranges[2 * t + 1] = (int32_t)(m + 1);
+}
+
+// --------------------------------------------------------------------------
+// Host-side plumbing
+// --------------------------------------------------------------------------
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+inline uint32_t nblk(uint64_t n) { return (uint32_t)((n + kTileItems - 1) / kTileItems); }
+
+// Workspace carve-up for a capacity of `cap` items of 32-bit keys + values.
+struct SortBufs {
+  uint32_t *keyA, *keyB;
+  int32_t *valA, *valB;
+  uint32_t *hist;      // [256][nb]
+  uint32_t *partials;  // scan partials (max of data scan and hist scan)
+};
+
+inline size_t sortbufs_bytes(uint64_t cap) {
+  const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
+  const uint64_t hn = (uint64_t)kRadix * nb;
+  const uint64_t pn = (nblk(cap) > nblk(hn) ? nblk(cap) : nblk(hn)) + 1;
+  return 4 * align256(cap * 4) + align256(hn * 4) + align256(pn * 4);
+}
+
+inline char *carve(char *p, size_t bytes, void **out) {
+  *out = p;
+  return p + align256(bytes);
+}
+
+inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b) {
+  const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
+  const uint64_t hn = (uint64_t)kRadix * nb;
+  const uint64_t pn = (nblk(cap) > nblk(hn) ? nblk(cap) : nblk(hn)) + 1;
+  p = carve(p, cap * 4, (void **)&b.keyA);
+  p = carve(p, cap * 4, (void **)&b.keyB);
+  p = carve(p, cap * 4, (void **)&b.valA);
+  p = carve(p, cap * 4, (void **)&b.valB);
+  p = carve(p, hn * 4, (void **)&b.hist);
+  p = carve(p, pn * 4, (void **)&b.partials);
+  return p;
+}
+
+template <class Load, class Store>
+void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap, uint32_t *partials,
+              uint32_t *total_out, uint32_t *total_out2, hipStream_t s) {
+  const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
+  scan_reduce_kernel<<<nb, kThreads, 0, s>>>(ld, n_ptr, cap, partials);
+  scan_partials_kernel<<<1, 1024, 0, s>>>(partials, nb, total_out, total_out2);
+  scan_down_kernel<<<nb, kThreads, 0, s>>>(ld, st, n_ptr, cap, partials);
+}
+
+// Stable LSD passes over bits [0, nbits) of 32-bit keys.  Data starts in
+// (keyA, valA) -- or in (keyA, v0) when v0 != nullptr -- and the final result
+// is returned through the pointers.
+void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
+               const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s) {
+  const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
+  const uint32_t hn = kRadix * nb;
+  for (int shift = 0; shift < nbits; shift += kRadixBits) {
+    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, b.hist, nb);
+    run_scan(LoadU32{b.hist}, StoreU32{b.hist}, nullptr, hn, b.partials, nullptr, nullptr, s);
+    radix_scatter_kernel<<<nb, kThreads, 0, s>>>(k_cur, v_cur, k_alt, v_alt, n_ptr, cap, shift,
+                                                 b.hist, nb);
+    uint32_t *kt = k_cur;
+    k_cur = k_alt;
+    k_alt = kt;
+    int32_t *vt = v_cur;
+    v_cur = v_alt;
+    v_alt = vt;
+  }
+}
+
+int tile_bits(int T) {
+  int b = 0;
+  while ((1 << b) < T) ++b;
+  return b;
+}
+
+// Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
+struct GaussWs {
+  uint32_t *cnt, *offs, *scalars;  // scalars[0] = Nv, scalars[1] = M
+  SortBufs sb;
+};
+
+inline size_t gauss_ws_bytes(uint64_t n) {
+  const uint64_t c = n ? n : 1;
+  return 2 * align256(c * 4) + align256(64) + sortbufs_bytes(c);
+}
+
+inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
+  const uint64_t c = n ? n : 1;
+  char *p = (char *)ws;
+  p = carve(p, c * 4, (void **)&g.cnt);
+  p = carve(p, c * 4, (void **)&g.offs);
+  p = carve(p, 64, (void **)&g.scalars);
+  carve_sortbufs(p, c, g.sb);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n > 0 ? n : 1)); }
+
+size_t gsr_sort_ws_inst_bytes(int64_t m) { return sortbufs_bytes((uint64_t)(m > 0 ? m : 1)); }
+
+int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                   int W, int H, int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
+                   void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
+              "sort_count: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  GaussWs g;
+  carve_gauss(ws_gauss, (uint64_t)n, g);
+  const uint32_t cap = (uint32_t)(n > 0 ? n : 1);
+  if (n == 0) {
+    fill_kernel<<<1, kThreads, 0, s>>>((int32_t *)g.scalars, 2, 0);
+    fill_kernel<<<1, kThreads, 0, s>>>(m_out, 1, 0);
+    return gsr_launch_status("sort_count");
+  }
+  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h};
+  count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
+  // compact visible splats with their depth bits as keys -> Nv in scalars[0]
+  run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
+           g.sb.partials, g.scalars, nullptr, s);
+  // depth sort of the visible splats (4 passes over 32 bits)
+  uint32_t *kc = g.sb.keyA;
+  int32_t *vc = g.sb.valA;
+  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s);
+  // offsets of each splat's instances, in depth order -> M in scalars[1]
+  run_scan(LoadSortedCount{g.cnt, vc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials,
+           g.scalars + 1, (uint32_t *)m_out, s);
+  return gsr_launch_status("sort_count");
+}
+
+int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
+                  int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                  int32_t *tile_range, void *stream) {
+  GSR_REQUIRE(n >= 0 && m >= 0 && m < (int64_t)INT32_MAX && W > 0 && H > 0,
+              "sort_emit: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+  const int T = gx * gy;
+  fill_kernel<<<gsr_blocks(2 * (int64_t)T, kThreads), kThreads, 0, s>>>(tile_range, 2 * (int64_t)T,
+                                                                        0);
+  if (m == 0 || n == 0) return gsr_launch_status("sort_emit");
+  GaussWs g;
+  carve_gauss(ws_gauss, (uint64_t)n, g);
+  SortBufs ib;
+  carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
+  const Grid gr{gx, gy, tile_w, tile_h};
+  const int nbits = tile_bits(T);
+  const int passes = (nbits + kRadixBits - 1) / kRadixBits;
+  // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
+  int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
+  int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;
+  emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, g.sb.valA, g.offs,
+                                                           g.scalars, (uint32_t)n, gr, ib.keyA,
+                                                           v_first);
+  uint32_t *kc = ib.keyA;
+  int32_t *vc = v_first;
+  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
+  ranges_kernel<<<gsr_blocks(m, kThreads), kThreads, 0, s>>>(kc, g.scalars + 1, (uint32_t)m,
+                                                             tile_range);
+  return gsr_launch_status("sort_emit");
+}
+
+}  // extern "C"

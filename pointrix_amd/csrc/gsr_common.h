@@ -1,0 +1,87 @@
+// Shared device helpers for the gfx950 Gaussian rasteriser.
+//
+// Every forward kernel is compiled with -ffp-contract=off: a float expression
+// is evaluated exactly as written (left to right, one rounding per operator)
+// and fused multiply-adds appear only where fmaf() is written.  Together with
+// correctly rounded '/' and sqrtf (HIP's default) and the exp below, built only
+// from IEEE-exact operations, the forward path is bit-reproducible by a plain
+// C program on the host that performs the same operations in the same order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsr {
+
+constexpr float kAlphaMax = 0.99f;
+constexpr float kAlphaMin = 1.0f / 255.0f;
+constexpr float kTMin = 1e-4f;
+constexpr float kLowPass = 0.3f;
+constexpr float kJClamp = 1.3f;
+// exp(-5.6) < 1/255 with a 6% margin: for opacity <= 1 a splat with
+// power < -5.6 cannot reach the alpha threshold, so exp can be skipped without
+// changing any decision.
+constexpr float kPowerCut = -5.6f;
+
+// exp(x) for x <= 0 (inputs below -80 are clamped; exp(-80) ~ 1.8e-35 is far
+// below any alpha threshold).  Cody-Waite reduction x = k*ln2 + r with ln2
+// split into a 15-bit head (k*head is exact for |k| < 2^8) and a tail, then
+// the degree-7 Taylor polynomial of e^r on |r| <= ln2/2 (truncation error
+// < 1e-8 relative), then an exact scaling by 2^k.  Uses only mul, fma, rint,
+// ldexp: every step is correctly rounded on the GPU and on the host alike.
+__device__ __forceinline__ float exp_neg(float x) {
+  x = fmaxf(x, -80.0f);
+  const float kf = rintf(x * 1.44269504088896341f);
+  float r = fmaf(kf, -0.693145751953125f, x);
+  r = fmaf(kf, -1.42860682030941723212e-6f, r);
+  float p = (float)(1.0 / 5040.0);
+  p = fmaf(p, r, (float)(1.0 / 720.0));
+  p = fmaf(p, r, (float)(1.0 / 120.0));
+  p = fmaf(p, r, (float)(1.0 / 24.0));
+  p = fmaf(p, r, (float)(1.0 / 6.0));
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)kf);
+}
+
+// Tile rectangle [x0,x1) x [y0,y1) covered by a splat of integer radius r at
+// (u, v): x0 = trunc((u - r)/tw), x1 = trunc((u + r + tw - 1)/tw), clamped to
+// the tile grid (clamping to >= 0 first makes the truncation a floor).
+struct Rect {
+  int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Rect tile_rect(float u, float v, int r, int gx, int gy, int tw,
+                                          int th) {
+  const float rf = (float)r;
+  const float ftw = (float)tw, fth = (float)th;
+  Rect q;
+  q.x0 = (int)fminf(fmaxf((u - rf) / ftw, 0.0f), (float)gx);
+  q.x1 = (int)fminf(fmaxf(((u + rf) + (ftw - 1.0f)) / ftw, 0.0f), (float)gx);
+  q.y0 = (int)fminf(fmaxf((v - rf) / fth, 0.0f), (float)gy);
+  q.y1 = (int)fminf(fmaxf(((v + rf) + (fth - 1.0f)) / fth, 0.0f), (float)gy);
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+}  // namespace gsr
+
+// Status codes returned through the C ABI.
+#define GSR_OK 0
+#define GSR_ERR_ARG -1000
+#define GSR_ERR_CAPACITY 1

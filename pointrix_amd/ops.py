@@ -1,0 +1,388 @@
+"""msplat-compatible differentiable ops backed by libgsr (HIP, gfx950).
+
+The six functions below have the names, positional arguments and return
+values of the msplat calls made by the reference renderer
+(pointrix/model/renderer/msplat.py:104-151, examples/supervise/renderer.py:42-86),
+so a renderer written against msplat ports by changing its import.  Each is a
+``torch.autograd.Function`` whose forward and backward are hand-written HIP
+kernels reached through the C ABI in include/gsr.h; tensors must live on the
+GPU -- there is no CPU path.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+
+TILE = 16  # msplat's tile edge (pixels); tile_range has ceil(W/16)*ceil(H/16) rows
+
+
+class KernelTimer:
+    """Optional per-op device timing: torch events recorded on the stream the
+    kernels are launched on, around each C-ABI call.  Enabled by the
+    benchmark; off (None) by default."""
+
+    def __init__(self):
+        self.events = {}
+
+    def span(self, name, dev):
+        return _Span(self, name, dev)
+
+    def summary(self):
+        """name -> (launches, total ms) after a device synchronisation."""
+        out = {}
+        for name, evs in self.events.items():
+            out[name] = (len(evs), sum(a.elapsed_time(b) for a, b in evs))
+        return out
+
+
+class _Span:
+    def __init__(self, timer, name, dev):
+        self.t, self.name, self.dev = timer, name, dev
+
+    def __enter__(self):
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.a.record(torch.cuda.current_stream(self.dev))
+
+    def __exit__(self, *exc):
+        b = torch.cuda.Event(enable_timing=True)
+        b.record(torch.cuda.current_stream(self.dev))
+        self.t.events.setdefault(self.name, []).append((self.a, b))
+
+
+timer: KernelTimer | None = None
+
+
+class _NoSpan:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
+
+
+def _span(name, dev):
+    return timer.span(name, dev) if timer is not None else _NoSpan()
+
+
+def _dev_check(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise RuntimeError("pointrix_amd ops run on the GPU only; got a CPU tensor")
+
+
+def _f32(t: torch.Tensor) -> torch.Tensor:
+    return t.detach().to(torch.float32).contiguous()
+
+
+def _intr4(intr: torch.Tensor) -> torch.Tensor:
+    t = _f32(intr).reshape(-1)
+    if t.numel() < 4:
+        raise ValueError("intrinsics must hold (fx, fy, cx, cy)")
+    return t[:4].contiguous()
+
+
+def _extr34(extr: torch.Tensor) -> torch.Tensor:
+    t = _f32(extr)
+    if t.shape == (4, 4):
+        t = t[:3].contiguous()
+    if t.shape != (3, 4):
+        raise ValueError(f"extrinsics must be [3,4] or [4,4], got {tuple(extr.shape)}")
+    return t
+
+
+def _intr_grad(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
+    out = torch.zeros(like.numel(), dtype=like.dtype, device=like.device)
+    out[:4] = g.to(like.dtype)
+    return out.reshape(like.shape)
+
+
+def _extr_grad(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
+    g = g.reshape(3, 4).to(like.dtype)
+    if like.shape == (4, 4):
+        g = torch.cat([g, torch.zeros(1, 4, dtype=g.dtype, device=g.device)], 0)
+    return g
+
+
+# ---------------------------------------------------------------------------
+# project_point  (msplat.py:109-113)
+# ---------------------------------------------------------------------------
+class _ProjectPoint(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, intr, extr, W, H, nearest, extent):
+        _dev_check(xyz, intr, extr)
+        x, k, e = _f32(xyz), _intr4(intr), _extr34(extr)
+        n = x.shape[0]
+        uv = torch.empty(n, 2, dtype=torch.float32, device=x.device)
+        depth = torch.empty(n, 1, dtype=torch.float32, device=x.device)
+        call("gsr_project_point_fwd", ptr(x), n, ptr(k), ptr(e), int(W), int(H), float(nearest),
+             float(extent), ptr(uv), ptr(depth), stream_of(x))
+        ctx.save_for_backward(x, k, e, depth)
+        ctx.shapes = (intr, extr)
+        return uv, depth
+
+    @staticmethod
+    def backward(ctx, g_uv, g_depth):
+        x, k, e, depth = ctx.saved_tensors
+        intr, extr = ctx.shapes
+        n = x.shape[0]
+        g_uv = torch.zeros(n, 2, device=x.device) if g_uv is None else _f32(g_uv)
+        g_d = None if g_depth is None else _f32(g_depth)
+        d_xyz = torch.empty_like(x)
+        need_k, need_e = ctx.needs_input_grad[1], ctx.needs_input_grad[2]
+        d_k = torch.zeros(4, device=x.device) if need_k else None
+        d_e = torch.zeros(12, device=x.device) if need_e else None
+        call("gsr_project_point_bwd", ptr(x), n, ptr(k), ptr(e), ptr(depth), ptr(g_uv), ptr(g_d),
+             ptr(d_xyz), ptr(d_k), ptr(d_e), stream_of(x))
+        return (d_xyz, _intr_grad(d_k, intr) if need_k else None,
+                _extr_grad(d_e, extr) if need_e else None, None, None, None, None)
+
+
+def project_point(xyz, intr, extr, W, H, nearest=0.2, extent=1.3):
+    """World points [N,3] -> (uv [N,2] pixels, depth [N,1]); culled points get 0."""
+    return _ProjectPoint.apply(xyz, intr, extr, W, H, nearest, extent)
+
+
+# ---------------------------------------------------------------------------
+# compute_cov3d  (msplat.py:118)
+# ---------------------------------------------------------------------------
+class _ComputeCov3d(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, scale, rot, visible):
+        _dev_check(scale, rot, visible)
+        s, q = _f32(scale), _f32(rot)
+        vis = visible.reshape(-1).to(torch.uint8).contiguous()
+        n = s.shape[0]
+        cov = torch.empty(n, 6, dtype=torch.float32, device=s.device)
+        call("gsr_compute_cov3d_fwd", ptr(s), ptr(q), ptr(vis), n, ptr(cov), stream_of(s))
+        ctx.save_for_backward(s, q, vis)
+        return cov
+
+    @staticmethod
+    def backward(ctx, g):
+        s, q, vis = ctx.saved_tensors
+        n = s.shape[0]
+        d_s = torch.empty_like(s)
+        d_q = torch.empty_like(q)
+        call("gsr_compute_cov3d_bwd", ptr(s), ptr(q), ptr(vis), n, ptr(_f32(g)), ptr(d_s),
+             ptr(d_q), stream_of(s))
+        return d_s, d_q, None
+
+
+def compute_cov3d(scale, rot, visible):
+    """(scale [N,3], unit quaternion wxyz [N,4], visible [N] or [N,1]) -> cov3d [N,6]."""
+    return _ComputeCov3d.apply(scale, rot, visible)
+
+
+# ---------------------------------------------------------------------------
+# ewa_project  (msplat.py:121-130)
+# ---------------------------------------------------------------------------
+class _EwaProject(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, cov3d, intr, extr, uv, W, H, visible):
+        _dev_check(xyz, cov3d, intr, extr, uv, visible)
+        x, c, k, e = _f32(xyz), _f32(cov3d), _intr4(intr), _extr34(extr)
+        u = _f32(uv)
+        vis = visible.reshape(-1).to(torch.uint8).contiguous()
+        n = x.shape[0]
+        conic = torch.empty(n, 3, dtype=torch.float32, device=x.device)
+        radius = torch.empty(n, dtype=torch.int32, device=x.device)
+        tiles = torch.empty(n, dtype=torch.int32, device=x.device)
+        call("gsr_ewa_project_fwd", ptr(x), ptr(c), n, ptr(k), ptr(e), ptr(u), int(W), int(H),
+             ptr(vis), ptr(conic), ptr(radius), ptr(tiles), stream_of(x))
+        ctx.save_for_backward(x, c, k, e, radius)
+        ctx.shapes = (intr, extr, int(W), int(H))
+        ctx.mark_non_differentiable(radius, tiles)
+        return conic, radius, tiles
+
+    @staticmethod
+    def backward(ctx, g_conic, _gr, _gt):
+        x, c, k, e, radius = ctx.saved_tensors
+        intr, extr, W, H = ctx.shapes
+        n = x.shape[0]
+        g = torch.zeros(n, 3, device=x.device) if g_conic is None else _f32(g_conic)
+        d_x = torch.empty_like(x)
+        d_c = torch.empty_like(c)
+        need_k, need_e = ctx.needs_input_grad[2], ctx.needs_input_grad[3]
+        d_k = torch.zeros(4, device=x.device) if need_k else None
+        d_e = torch.zeros(12, device=x.device) if need_e else None
+        call("gsr_ewa_project_bwd", ptr(x), ptr(c), n, ptr(k), ptr(e), W, H, ptr(radius), ptr(g),
+             ptr(d_x), ptr(d_c), ptr(d_k), ptr(d_e), stream_of(x))
+        return (d_x, d_c, _intr_grad(d_k, intr) if need_k else None,
+                _extr_grad(d_e, extr) if need_e else None, None, None, None, None)
+
+
+def ewa_project(xyz, cov3d, intr, extr, uv, W, H, visible):
+    """-> (conic [N,3], radius [N] int32, tiles touched [N] int32)."""
+    return _EwaProject.apply(xyz, cov3d, intr, extr, uv, W, H, visible)
+
+
+# ---------------------------------------------------------------------------
+# sort_gaussian  (msplat.py:133-135) -- not differentiable
+# ---------------------------------------------------------------------------
+class SortWorkspace:
+    """Reusable device workspace for the binning stage (grows, never shrinks)."""
+
+    def __init__(self):
+        self.gauss = None
+        self.inst = None
+
+    def gauss_buf(self, n, device):
+        need = int(_lib.load().gsr_sort_ws_gauss_bytes(n))
+        if self.gauss is None or self.gauss.numel() < need or self.gauss.device != device:
+            self.gauss = torch.empty(need, dtype=torch.uint8, device=device)
+        return self.gauss
+
+    def inst_buf(self, m, device):
+        need = int(_lib.load().gsr_sort_ws_inst_bytes(m))
+        if self.inst is None or self.inst.numel() < need or self.inst.device != device:
+            self.inst = torch.empty(need, dtype=torch.uint8, device=device)
+        return self.inst
+
+
+_default_ws = SortWorkspace()
+last_instance_count = 0  # M of the most recent sort_gaussian call (diagnostics)
+
+
+def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, workspace=None):
+    """-> (gaussian_ids_sorted [M] int32, tile_range [T,2] int32).
+
+    Order: (tile, float32 bits of depth, splat index).  One host read of the
+    instance count M sizes the outputs.  ``tiles`` is accepted for msplat
+    signature compatibility; counts are recomputed from (uv, radius), which is
+    what ewa_project's ``tiles`` holds.
+    """
+    _dev_check(uv, depth, radius)
+    ws = workspace or _default_ws
+    u = _f32(uv)
+    d = _f32(depth).reshape(-1)
+    r = radius.reshape(-1).to(torch.int32).contiguous()
+    n = u.shape[0]
+    dev = u.device
+    st = stream_of(u)
+    gx, gy = (W + tile_w - 1) // tile_w, (H + tile_h - 1) // tile_h
+    m_dev = torch.empty(1, dtype=torch.int32, device=dev)
+    g = ws.gauss_buf(n, dev)
+    with _span("sort_count", dev):
+        call("gsr_sort_count", ptr(u), ptr(d), ptr(r), n, int(W), int(H), tile_w, tile_h,
+             ptr(g), ptr(m_dev), st)
+    m = int(m_dev.item())
+    global last_instance_count
+    last_instance_count = m
+    ids = torch.empty(m, dtype=torch.int32, device=dev)
+    tile_range = torch.empty(gx * gy, 2, dtype=torch.int32, device=dev)
+    inst = ws.inst_buf(m, dev)
+    with _span("sort_emit", dev):
+        call("gsr_sort_emit", ptr(u), ptr(r), n, int(W), int(H), tile_w, tile_h, ptr(g), m,
+             ptr(inst), ptr(ids), ptr(tile_range), st)
+    return ids, tile_range
+
+
+# ---------------------------------------------------------------------------
+# alpha_blending  (msplat.py:148-151)
+# ---------------------------------------------------------------------------
+class _AlphaBlending(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc, tile_w, tile_h):
+        _dev_check(uv, conic, opacity, feature, ids, tile_range)
+        u, c, o, f = _f32(uv), _f32(conic), _f32(opacity).reshape(-1), _f32(feature)
+        ids_c = ids.to(torch.int32).contiguous()
+        tr = tile_range.to(torch.int32).contiguous()
+        C = f.shape[1]
+        if not 1 <= C <= 16:
+            raise ValueError(f"alpha_blending supports 1..16 channels, got {C}")
+        dev = u.device
+        out = torch.empty(C, H, W, dtype=torch.float32, device=dev)
+        T = torch.empty(H, W, dtype=torch.float32, device=dev)
+        nc = torch.empty(H, W, dtype=torch.int32, device=dev)
+        with _span("alpha_blending_fwd", dev):
+            call("gsr_alpha_blending_fwd", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids_c),
+                 ptr(tr), float(bg), int(W), int(H), tile_w, tile_h, ptr(out), ptr(T), ptr(nc),
+                 stream_of(u))
+        ctx.save_for_backward(u, c, o, f, ids_c, tr, T, nc)
+        ctx.cfg = (float(bg), int(W), int(H), tile_w, tile_h, opacity.shape)
+        ctx.mark_non_differentiable(nc)
+        return out, T, nc
+
+    @staticmethod
+    def backward(ctx, g_out, g_T, _g_nc):
+        u, c, o, f, ids, tr, T, nc = ctx.saved_tensors
+        bg, W, H, tw, th, op_shape = ctx.cfg
+        n, C = f.shape
+        dev = u.device
+        g_out = torch.zeros(C, H, W, device=dev) if g_out is None else _f32(g_out)
+        g_T = None if g_T is None else _f32(g_T)
+        d_u = torch.zeros(n, 2, device=dev)
+        d_c = torch.zeros(n, 3, device=dev)
+        d_o = torch.zeros(n, device=dev)
+        d_f = torch.zeros(n, C, device=dev)
+        with _span("alpha_blending_bwd", dev):
+            call("gsr_alpha_blending_bwd", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids), ptr(tr),
+                 bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(d_u), ptr(d_c),
+                 ptr(d_o), ptr(d_f), stream_of(u))
+        d_ndc = None
+        if ctx.needs_input_grad[9]:
+            d_ndc = d_u * torch.tensor([0.5 * W, 0.5 * H], device=dev)
+        return (d_u, d_c, d_o.reshape(op_shape), d_f, None, None, None, None, None, d_ndc, None,
+                None)
+
+
+def alpha_blending_ex(uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc=None,
+                      tile_w=TILE, tile_h=TILE):
+    """alpha_blending that also returns T_final [H,W] (differentiable) and
+    n_contrib [H,W] int32; 1 - T_final is the rendered opacity."""
+    return _AlphaBlending.apply(uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc,
+                                tile_w, tile_h)
+
+
+def alpha_blending(uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc=None):
+    """Front-to-back compositing of feature [N,C] -> [C,H,W].
+
+    ``ndc`` (optional, [N,2], a zero leaf with requires_grad) receives
+    dL/duv * (W/2, H/2), the viewspace-gradient the densification controller
+    reads (pointrix/controller/gs.py:259-284); its value is not used.
+    """
+    out, _, _ = alpha_blending_ex(uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# compute_sh  (msplat.py:104)
+# ---------------------------------------------------------------------------
+class _ComputeSH(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, shs, dirs, degree):
+        _dev_check(shs, dirs)
+        s = shs.detach()
+        if s.dtype != torch.float32:
+            s = s.float()
+        d = _f32(dirs)
+        n, C, K = s.shape
+        out = torch.empty(n, C, dtype=torch.float32, device=d.device)
+        sn, sc, sk = s.stride()
+        call("gsr_compute_sh_fwd", ptr(s), sn, sc, sk, ptr(d), n, C, K, int(degree), ptr(out),
+             stream_of(d))
+        ctx.save_for_backward(s, d)
+        ctx.degree = int(degree)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        s, d = ctx.saved_tensors
+        n, C, K = s.shape
+        d_s = torch.empty(n, C, K, dtype=torch.float32, device=d.device)
+        d_d = torch.empty_like(d) if ctx.needs_input_grad[1] else None
+        sn, sc, sk = s.stride()
+        call("gsr_compute_sh_bwd", ptr(s), sn, sc, sk, ptr(d), n, C, K, ctx.degree, ptr(_f32(g)),
+             ptr(d_s), ptr(d_d), stream_of(d))
+        return d_s, d_d, None
+
+
+def compute_sh(shs, dirs, degree=None):
+    """Real-SH evaluation: shs [N,C,K] (any strides), unit dirs [N,3] -> [N,C].
+
+    ``degree`` restricts the sum to the first (degree+1)^2 coefficients (the
+    SH warm-up of msplat.py:98-101 without materialising a mask).
+    """
+    return _ComputeSH.apply(shs, dirs, -1 if degree is None else degree)

@@ -1,0 +1,240 @@
+"""MsplatRender-compatible renderer plugin running on libgsr.
+
+Mirrors pointrix/model/renderer/msplat.py (class MsplatRender, its Config,
+setup, render_iter, render_batch, update_sh_degree, state_dict) and
+pointrix/model/renderer/utils/renderer_utils.py (RenderFeatures), with the
+msplat calls replaced by ``pointrix_amd.ops``.  Same inputs, same output dict
+("rendered_features_split", "uv_points", "visibility", "radii"; render_batch
+stacks features per name).  Extensions, all off by default so the default
+output keys equal the reference's: ``render_normal`` (per-splat normal channel,
+as examples/supervise renders it) and ``render_opacity`` (1 - T_final).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, fields
+from typing import Dict
+
+import torch
+
+from . import ops
+
+
+class Registry:
+    """Name -> class registry (the role of pointrix/utils/registry.py)."""
+
+    def __init__(self, name):
+        self.name = name
+        self._map = {}
+
+    def register(self, obj=None):
+        def deco(o):
+            self._map[o.__name__] = o
+            return o
+        return deco(obj) if obj is not None else deco
+
+    def get(self, name):
+        if name not in self._map:
+            raise KeyError(f"{name} is not registered in {self.name}")
+        return self._map[name]
+
+    def __contains__(self, name):
+        return name in self._map
+
+
+RENDERER_REGISTRY = Registry("RENDERER")
+
+
+class RenderFeatures:
+    """Per-splat feature channels concatenated for one blend, split afterwards
+    (renderer_utils.py:4-71); insertion order defines the channel order."""
+
+    def __init__(self, **kwargs) -> None:
+        for k, v in kwargs.items():
+            setattr(self, k, v)
+
+    def to(self, device: str) -> None:
+        for k, v in self.__dict__.items():
+            if isinstance(v, torch.Tensor):
+                setattr(self, k, v.to(device))
+
+    def combine(self) -> torch.Tensor:
+        return torch.cat([v for v in self.__dict__.values() if isinstance(v, torch.Tensor)], -1)
+
+    def split(self, feature: torch.Tensor) -> Dict[str, torch.Tensor]:
+        out, start = {}, 0
+        for k, v in self.__dict__.items():
+            if isinstance(v, torch.Tensor):
+                end = start + v.shape[-1]
+                out[k] = feature[start:end, ...]
+                start = end
+        return out
+
+
+def quat_to_rotmat(q: torch.Tensor) -> torch.Tensor:
+    """wxyz quaternion -> rotation matrix (pointrix/utils/pose.py:40-84)."""
+    w, x, y, z = q.unbind(-1)
+    x2, y2, z2, w2 = x * x, y * y, z * z, w * w
+    xy, zw, xz, yw, yz, xw = x * y, z * w, x * z, y * w, y * z, x * w
+    return torch.stack([
+        torch.stack([x2 - y2 - z2 + w2, 2 * (xy - zw), 2 * (xz + yw)], -1),
+        torch.stack([2 * (xy + zw), -x2 + y2 - z2 + w2, 2 * (yz - xw)], -1),
+        torch.stack([2 * (xz - yw), 2 * (yz + xw), -x2 - y2 + z2 + w2], -1)], -2)
+
+
+def splat_normals(position, scaling, rotation, camera_center, extrinsic_matrix):
+    """Camera-space normal of each splat: its shortest axis, turned towards the
+    camera (examples/supervise/model.py:50-74)."""
+    R = quat_to_rotmat(rotation)
+    idx = torch.argmin(scaling.detach(), dim=-1)
+    n = torch.gather(R, 2, idx[:, None, None].expand(-1, 3, 1)).squeeze(-1)
+    n = torch.nn.functional.normalize(n, dim=-1)
+    d = (camera_center.reshape(1, 3) - position).detach()
+    flip = (d * n.detach()).sum(-1) < 0
+    n = torch.where(flip[:, None], -n, n)
+    return n @ extrinsic_matrix[:3, :3].T
+
+
+@RENDERER_REGISTRY.register()
+class MsplatRender:
+    """Differentiable Gaussian rasteriser plugin (msplat.py:12-248)."""
+
+    @dataclass
+    class Config:
+        update_sh_iter: int = 1000
+        max_sh_degree: int = 3
+        render_depth: bool = False
+        render_normal: bool = False
+        render_opacity: bool = False
+
+    def __init__(self, cfg=None, white_bg=False, device="cuda", **kwargs):
+        if cfg is None:
+            cfg = self.Config()
+        elif isinstance(cfg, dict):
+            known = {f.name for f in fields(self.Config)}
+            cfg = self.Config(**{k: v for k, v in cfg.items() if k in known})
+        self.cfg = cfg
+        self.setup(white_bg, device, **kwargs)
+
+    def setup(self, white_bg, device, **kwargs):
+        self.sh_degree = 0
+        self.device = device
+        self.bg_color = 1.0 if white_bg else 0.0
+        self.workspace = ops.SortWorkspace()
+        # when True, render_iter keeps its per-splat / per-pixel intermediates
+        # (uv, depth, conic, radius, ids, tile_range, features, T_final,
+        # n_contrib) in self.intermediates, for parity checks and debugging
+        self.keep_intermediates = False
+        self.intermediates = None
+
+    def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
+                    position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        direction = position - camera_center.reshape(1, 3)
+        direction = direction / direction.norm(dim=1, keepdim=True)
+        # SH warm-up: the first (sh_degree+1)^2 coefficients (msplat.py:98-101)
+        rgb = ops.compute_sh(shs.permute(0, 2, 1), direction, degree=self.sh_degree)
+        rgb = (rgb + 0.5).clamp(min=0.0)
+        extr = extrinsic_matrix[:3, :]
+        uv, depth = ops.project_point(position, intrinsic_params, extr, width, height,
+                                      nearest=0.2)
+        visible = depth != 0
+        cov3d = ops.compute_cov3d(scaling, rotation, visible)
+        conic, radius, tiles = ops.ewa_project(position, cov3d, intrinsic_params, extr, uv, width,
+                                               height, visible)
+        ids, tile_range = ops.sort_gaussian(uv, depth, width, height, radius, tiles,
+                                            workspace=self.workspace)
+        feats = {"rgb": rgb}
+        if self.cfg.render_depth:
+            feats["depth"] = depth
+        if normals is None and self.cfg.render_normal:
+            normals = splat_normals(position, scaling, rotation, camera_center, extrinsic_matrix)
+        if normals is not None:
+            feats["normal"] = normals
+        Render_Features = RenderFeatures(**feats)
+        render_features = Render_Features.combine()
+        ndc = torch.zeros_like(uv, requires_grad=True)
+        ndc.retain_grad()
+        rendered, T_final, n_contrib = ops.alpha_blending_ex(uv, conic, opacity, render_features,
+                                                             ids, tile_range, self.bg_color,
+                                                             width, height, ndc)
+        if self.keep_intermediates:
+            self.intermediates = {"uv": uv, "depth": depth, "cov3d": cov3d, "conic": conic,
+                                  "radius": radius, "ids": ids, "tile_range": tile_range,
+                                  "features": render_features, "rendered": rendered,
+                                  "T_final": T_final, "n_contrib": n_contrib}
+        split = Render_Features.split(rendered)
+        if "normal" in split:
+            # [-1,1] normals to [0,1] (examples/supervise/renderer.py:89-96)
+            nrm = split["normal"]
+            split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        if self.cfg.render_opacity:
+            split["opacity"] = (1.0 - T_final)[None]
+        return {"rendered_features_split": split, "uv_points": ndc,
+                "visibility": radius > 0, "radii": radius}
+
+    def render_batch(self, render_dict: dict, batch=None) -> dict:
+        """Per-view loop of msplat.py:160-213 (extrinsics and centres per view;
+        intrinsics and image size shared)."""
+        rendered, uv_points, vis, radii = {}, [], [], []
+        per_view = ["extrinsic_matrix", "camera_center"]
+        for i in range(render_dict["extrinsic_matrix"].shape[0]):
+            kw = {k: (v[i, ...] if k in per_view else v) for k, v in render_dict.items()}
+            res = self.render_iter(**kw)
+            for name, img in res["rendered_features_split"].items():
+                rendered.setdefault(name, []).append(img)
+            uv_points.append(res["uv_points"])
+            vis.append(res["visibility"].unsqueeze(0))
+            radii.append(res["radii"].unsqueeze(0))
+        for name in rendered:
+            rendered[name] = torch.stack(rendered[name], dim=0)
+        return {**rendered, "uv_points": uv_points,
+                "visibility": torch.cat(vis).any(dim=0),
+                "radii": torch.cat(radii, 0).max(dim=0).values}
+
+    def update_sh_degree(self, step):
+        if step % self.cfg.update_sh_iter == 0 and self.sh_degree < self.cfg.max_sh_degree:
+            self.sh_degree += 1
+
+    def load_state_dict(self, state_dict):
+        self.sh_degree = state_dict["sh_degree"]
+
+    def state_dict(self):
+        return {"sh_degree": self.sh_degree}
+
+
+@RENDERER_REGISTRY.register()
+class MsplatNormalRender(MsplatRender):
+    """Normal-supervision variant (examples/supervise/renderer.py:8-102): blends
+    the ``normals`` kwarg without the +0.5 colour offset."""
+
+    def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
+                    position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        direction = position - camera_center.reshape(1, 3)
+        direction = direction / direction.norm(dim=1, keepdim=True)
+        rgb = ops.compute_sh(shs.permute(0, 2, 1), direction)
+        extr = extrinsic_matrix[:3, :]
+        uv, depth = ops.project_point(position, intrinsic_params, extr, width, height)
+        visible = depth != 0
+        cov3d = ops.compute_cov3d(scaling, rotation, visible)
+        conic, radius, tiles = ops.ewa_project(position, cov3d, intrinsic_params, extr, uv, width,
+                                               height, visible)
+        ids, tile_range = ops.sort_gaussian(uv, depth, width, height, radius, tiles,
+                                            workspace=self.workspace)
+        Render_Features = RenderFeatures(rgb=rgb, normal=normals) if normals is not None \
+            else RenderFeatures(rgb=rgb)
+        ndc = torch.zeros_like(uv, requires_grad=True)
+        ndc.retain_grad()
+        rendered = ops.alpha_blending(uv, conic, opacity, Render_Features.combine(), ids,
+                                      tile_range, self.bg_color, width, height, ndc)
+        split = Render_Features.split(rendered)
+        if normals is not None:
+            nrm = split["normal"]
+            split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        return {"rendered_features_split": split, "uv_points": ndc,
+                "visibility": radius > 0, "radii": radius}
+
+
+def parse_renderer(cfg, **kwargs):
+    """Build a renderer from a config carrying ``name`` (renderer/__init__.py:4-19)."""
+    cfg = dict(cfg)
+    name = cfg.pop("name", "MsplatRender")
+    return RENDERER_REGISTRY.get(name)(cfg, **kwargs)
