@@ -78,6 +78,32 @@ int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
                        const float *dirs, int64_t n, int C, int K, int degree,
                        const float *d_out, float *d_shs, float *d_dirs, void *stream);
 
+/* ---- fused preprocess of MsplatRender.render_iter (msplat.py:94-139) -----
+ * One pass per splat replacing the view-direction normalisation (:94-96),
+ * the SH warm-up mask + compute_sh + "+0.5, clamp" (:98-105), project_point,
+ * compute_cov3d, ewa_project (:107-130) and RenderFeatures.combine (:137-139).
+ * shs: [N,K,3] (K in 1,4,9,16), summed over the first min(K,(degree+1)^2)
+ * coefficients (degree < 0: all).  flags: 1 = depth channel, 2 = camera-space
+ * normal channel (shortest splat axis facing the camera,
+ * examples/supervise/model.py:50-74).  feat[N,C], C = 3 (+1) (+3):
+ * rgb | depth | normal.  uv/depth/conic/radius/tiles equal those of the
+ * separate ops; cov3d[N,6] may be NULL. */
+int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
+                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *cam_center, int W, int H, float nearest, float extent,
+                       int flags, float *uv, float *depth, float *cov3d, float *conic,
+                       int32_t *radius, int32_t *tiles, float *feat, void *stream);
+/* Backward: d_uv[N,2], d_depth[N] (may be NULL), d_conic[N,3], d_feat[N,C]
+ * -> d_xyz[N,3], d_scale[N,3], d_rot[N,4], d_shs[N,K,3] (all overwritten),
+ * d_intr[4], d_extr[12], d_center[3] (accumulated; each may be NULL). */
+int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
+                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *cam_center, int W, int H, float nearest, float extent,
+                       int flags, const int32_t *radius, const float *d_uv, const float *d_depth,
+                       const float *d_conic, const float *d_feat, float *d_xyz, float *d_scale,
+                       float *d_rot, float *d_shs, float *d_intr, float *d_extr,
+                       float *d_center, void *stream);
+
 /* ---- msplat.sort_gaussian (msplat.py:133-135) ----------------------------
  * Two phases around one host read of M (the instance count):
  *   gsr_sort_count: per-splat tile counts from (uv, radius), depth-sort of

@@ -321,3 +321,68 @@ void ref_blend(const float *uv, const float *conic, const float *opacity, const 
       ncon[(int64_t)py * W + px] = last;
     }
 }
+
+/* ---- fused preprocess (direction, SH colour, project, cov3d, EWA, feature
+ * pack) in the fused kernel's order; feat = rgb | depth (flag 1) | normal
+ * (flag 2) ---------------------------------------------------------------- */
+void ref_preprocess(const float *xyz, const float *scale, const float *rot, const float *shs,
+                    int64_t n, int K, int degree, const float *Kin, const float *E,
+                    const float *center, int W, int H, float nearest, float extent, int flags,
+                    float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                    float *feat) {
+  const int C = 3 + ((flags & 1) ? 1 : 0) + ((flags & 2) ? 3 : 0);
+  int Ke = K;
+  uint8_t one = 1;
+  if (degree >= 0 && (degree + 1) * (degree + 1) < K) Ke = (degree + 1) * (degree + 1);
+  for (int64_t i = 0; i < n; ++i) {
+    float cov[6], f[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float *x = xyz + 3 * i;
+    int c;
+    ref_project(x, 1, Kin, E, W, H, nearest, extent, uv + 2 * i, depth + i);
+    conic[3 * i] = conic[3 * i + 1] = conic[3 * i + 2] = 0.0f;
+    radius[i] = 0;
+    tiles[i] = 0;
+    if (depth[i] != 0.0f) {
+      ref_cov3d(scale + 3 * i, rot + 4 * i, &one, 1, cov);
+      ref_ewa(x, cov, 1, Kin, E, uv + 2 * i, W, H, &one, conic + 3 * i, radius + i, tiles + i);
+    }
+    if (radius[i] > 0) {
+      const float dx = x[0] - center[0], dy = x[1] - center[1], dz = x[2] - center[2];
+      const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+      float dir[3], out3[3];
+      int o = 3;
+      dir[0] = dx / len;
+      dir[1] = dy / len;
+      dir[2] = dz / len;
+      {
+        /* per-channel SH sum over the first Ke coefficients of [K,3] */
+        float tmp[16 * 3];
+        int k;
+        for (c = 0; c < 3; ++c)
+          for (k = 0; k < K; ++k) tmp[c * K + k] = shs[(i * K + k) * 3 + c];
+        ref_sh(tmp, dir, 1, 3, K, Ke == K ? -1 : degree, out3);
+      }
+      for (c = 0; c < 3; ++c) f[c] = fmaxf(out3[c] + 0.5f, 0.0f);
+      if (flags & 1) f[o++] = depth[i];
+      if (flags & 2) {
+        float R[9], m[3], nh[3], len2, inv, dot, sgn, smin;
+        const float *sc = scale + 3 * i;
+        int k = 0, j;
+        rotmat(rot + 4 * i, R);
+        smin = sc[0];
+        if (sc[1] < smin) { k = 1; smin = sc[1]; }
+        if (sc[2] < smin) { k = 2; }
+        m[0] = R[k];
+        m[1] = R[3 + k];
+        m[2] = R[6 + k];
+        len2 = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        inv = 1.0f / fmaxf(len2, 1e-12f);
+        dot = (center[0] - x[0]) * m[0] + (center[1] - x[1]) * m[1] + (center[2] - x[2]) * m[2];
+        sgn = dot < 0.0f ? -1.0f : 1.0f;
+        for (j = 0; j < 3; ++j) nh[j] = sgn * (m[j] * inv);
+        for (j = 0; j < 3; ++j) f[o + j] = E[4 * j] * nh[0] + E[4 * j + 1] * nh[1] + E[4 * j + 2] * nh[2];
+      }
+    }
+    for (c = 0; c < C; ++c) feat[i * C + c] = f[c];
+  }
+}

@@ -33,6 +33,8 @@ def lib():
         L.ref_sort.restype = I64
         L.ref_sort.argtypes = [P, P, P, I64, I, I, I, I, P, P]
         L.ref_blend.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P]
+        L.ref_preprocess.argtypes = [P, P, P, P, I64, I, I, P, P, P, I, I, F, F, I, P, P, P, P, P,
+                                     P]
         _lib = L
     return _lib
 
@@ -119,3 +121,26 @@ def alpha_blending(uv, conic, opacity, feature, ids, tile_range, bg, W, H, tile_
     lib().ref_blend(_p(u), _p(c), _p(o), _p(f), C, _p(i), _p(r), bg, W, H, tile_w, tile_h, _p(out),
                     _p(T), _p(nc))
     return out, T, nc
+
+
+def preprocess(xyz, scale, rot, shs, intr, extr, center, W, H, degree=-1, flags=0, nearest=0.2,
+               extent=1.3):
+    """Fused preprocess restatement: shs [N,K,3] -> (uv, depth [N,1], conic, radius, tiles,
+    feat [N,C]) with C = 3 (+1 if flags & 1) (+3 if flags & 2)."""
+    x, s, q = _c(xyz, np.float32), _c(scale, np.float32), _c(rot, np.float32)
+    sh = _c(shs, np.float32)
+    K = _c(np.asarray(intr).reshape(-1)[:4], np.float32)
+    E = _c(np.asarray(extr)[:3, :4], np.float32)
+    cen = _c(np.asarray(center).reshape(-1)[:3], np.float32)
+    n, Kc = x.shape[0], sh.shape[1]
+    C = 3 + (1 if flags & 1 else 0) + (3 if flags & 2 else 0)
+    uv = np.zeros((n, 2), np.float32)
+    d = np.zeros((n, 1), np.float32)
+    conic = np.zeros((n, 3), np.float32)
+    radius = np.zeros(n, np.int32)
+    tiles = np.zeros(n, np.int32)
+    feat = np.zeros((n, C), np.float32)
+    lib().ref_preprocess(_p(x), _p(s), _p(q), _p(sh), n, Kc, -1 if degree is None else degree,
+                         _p(K), _p(E), _p(cen), W, H, nearest, extent, flags, _p(uv), _p(d),
+                         _p(conic), _p(radius), _p(tiles), _p(feat))
+    return uv, d, conic, radius, tiles, feat

@@ -1,0 +1,380 @@
+// Fused per-splat preprocessing for MsplatRender.render_iter
+// (pointrix/model/renderer/msplat.py:94-139): view direction, SH colour
+// (+0.5, clamped at 0), project_point, compute_cov3d, ewa_project and the
+// blend feature pack (rgb | depth | camera-space normal) in one pass over the
+// splats, and the matching backward that chains every vector-Jacobian
+// product in registers.  Same per-op arithmetic (gsr_math.h) as the separate
+// msplat ops, so uv / depth / cov3d / conic / radius are identical to theirs.
+//
+// Layouts: shs is the [N, KS, 3] array render_iter receives (coefficient
+// major, channel minor); only the first KE = min(KS, (degree+1)^2)
+// coefficients are read.  feat is [N, C] with C = 3 (+1 depth) (+3 normal).
+#include "gsr_internal.h"
+#include "gsr_math.h"
+
+using namespace gsr;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kFlagDepth = 1;
+constexpr int kFlagNormal = 2;
+
+// Load the first KE coefficients (x3 channels) of splat i.
+template <int KS, int KE>
+__device__ __forceinline__ void load_sh(const float *__restrict__ shs, int64_t i,
+                                        float (&sh)[KE * 3]) {
+  const float *base = shs + i * (KS * 3);
+  if constexpr ((KS * 3) % 4 == 0) {
+    constexpr int n4 = (KE * 3 + 3) / 4;
+    const float4 *b4 = reinterpret_cast<const float4 *>(base);
+#pragma unroll
+    for (int j = 0; j < n4; ++j) {
+      const float4 t = b4[j];
+      if (4 * j + 0 < KE * 3) sh[4 * j + 0] = t.x;
+      if (4 * j + 1 < KE * 3) sh[4 * j + 1] = t.y;
+      if (4 * j + 2 < KE * 3) sh[4 * j + 2] = t.z;
+      if (4 * j + 3 < KE * 3) sh[4 * j + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KE * 3; ++j) sh[j] = base[j];
+  }
+}
+
+__device__ __forceinline__ float3 view_dir(float3 x, const float (&c)[3], float &len) {
+  const float dx = x.x - c[0], dy = x.y - c[1], dz = x.z - c[2];
+  len = sqrtf(dx * dx + dy * dy + dz * dz);
+  return make_float3(dx / len, dy / len, dz / len);
+}
+
+template <int KS, int KE>
+__global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
+    const float *__restrict__ xyz, const float *__restrict__ scale,
+    const float *__restrict__ rot, const float *__restrict__ shs, int64_t n,
+    const float *__restrict__ intr, const float *__restrict__ extr,
+    const float *__restrict__ center, int W, int H, int gx, int gy, float nearest,
+    float extent, int flags, int C, float *__restrict__ uv, float *__restrict__ depth,
+    float *__restrict__ cov_out, float *__restrict__ conic, int32_t *__restrict__ radius,
+    int32_t *__restrict__ tiles, float *__restrict__ feat) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CamV cam = cam_load(intr, extr);
+  const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  float u, v, d;
+  project(cam, x, nearest, extent, W, H, u, v, d);
+  uv[2 * i] = u;
+  uv[2 * i + 1] = v;
+  depth[i] = d;
+  float con[3] = {0.f, 0.f, 0.f};
+  float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int r = 0, t = 0;
+  if (d != 0.0f) {
+    const float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
+    const float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    covariance3d(s, q, cov);
+    EwaFwd e;
+    ewa_forward(cam, x, cov, W, H, e);
+    int rr;
+    float cc[3];
+    if (conic_radius(e.a, e.b, e.c, cc, rr)) {
+      const Rect qr = tile_rect(u, v, rr, gx, gy, 16, 16);
+      const int area = (qr.x1 - qr.x0) * (qr.y1 - qr.y0);
+      if (area > 0) {
+        r = rr;
+        t = area;
+        con[0] = cc[0];
+        con[1] = cc[1];
+        con[2] = cc[2];
+      }
+    }
+    if (r > 0) {
+      const float cen[3] = {center[0], center[1], center[2]};
+      float len;
+      const float3 dir = view_dir(x, cen, len);
+      float Y[16];
+      sh_basis(dir, KE, Y);
+      float sh[KE * 3];
+      load_sh<KS, KE>(shs, i, sh);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float acc = sh[c] * Y[0];
+#pragma unroll
+        for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
+        f[c] = fmaxf(acc + 0.5f, 0.0f);
+      }
+      int o = 3;
+      if (flags & kFlagDepth) f[o++] = d;
+      if (flags & kFlagNormal) {
+        NormalFwd nf;
+        float nc[3];
+        splat_normal(s, q, x, cen, cam, nf, nc);
+        f[o] = nc[0];
+        f[o + 1] = nc[1];
+        f[o + 2] = nc[2];
+      }
+    }
+  }
+  if (cov_out != nullptr) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cov_out[6 * i + k] = cov[k];
+  }
+  conic[3 * i] = con[0];
+  conic[3 * i + 1] = con[1];
+  conic[3 * i + 2] = con[2];
+  radius[i] = r;
+  tiles[i] = t;
+  for (int c = 0; c < C; ++c) feat[i * C + c] = f[c];
+}
+
+template <int KS, int KE>
+__global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
+    const float *__restrict__ xyz, const float *__restrict__ scale,
+    const float *__restrict__ rot, const float *__restrict__ shs, int64_t n,
+    const float *__restrict__ intr, const float *__restrict__ extr,
+    const float *__restrict__ center, int W, int H, float nearest, float extent, int flags,
+    int C, const int32_t *__restrict__ radius, const float *__restrict__ d_uv,
+    const float *__restrict__ d_depth, const float *__restrict__ d_conic,
+    const float *__restrict__ d_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
+    float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_intr,
+    float *__restrict__ d_extr, float *__restrict__ d_center) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float gcam[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) gcam[k] = 0.0f;
+  float gcen[3] = {0.f, 0.f, 0.f};
+  if (i < n) {
+    const CamV cam = cam_load(intr, extr);
+    float3 gx = make_float3(0.f, 0.f, 0.f);
+    float gs[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f};
+    float dsh[KE * 3];
+#pragma unroll
+    for (int j = 0; j < KE * 3; ++j) dsh[j] = 0.0f;
+    const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    if (radius[i] > 0) {
+      const float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
+      const float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+      const float cen[3] = {center[0], center[1], center[2]};
+      // colour: SH -> +0.5 -> clamp(min=0) (gradient passes where x >= 0)
+      float len;
+      const float3 dir = view_dir(x, cen, len);
+      float Y[16];
+      sh_basis(dir, KE, Y);
+      float sh[KE * 3];
+      load_sh<KS, KE>(shs, i, sh);
+      float coef[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) coef[k] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float acc = sh[c] * Y[0];
+#pragma unroll
+        for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
+        const float g = (acc + 0.5f >= 0.0f) ? d_feat[i * C + c] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+          dsh[3 * k + c] = g * Y[k];
+          coef[k] += g * sh[3 * k + c];
+        }
+      }
+      if (KE > 1) {
+        float3 gdir = make_float3(0.f, 0.f, 0.f);
+        sh_basis_vjp(dir, KE, coef, gdir);
+        const float dp = dir.x * gdir.x + dir.y * gdir.y + dir.z * gdir.z;
+        const float il = 1.0f / len;
+        const float3 gd = make_float3((gdir.x - dir.x * dp) * il, (gdir.y - dir.y * dp) * il,
+                                      (gdir.z - dir.z * dp) * il);
+        gx.x += gd.x;
+        gx.y += gd.y;
+        gx.z += gd.z;
+        gcen[0] -= gd.x;
+        gcen[1] -= gd.y;
+        gcen[2] -= gd.z;
+      }
+      int o = 3;
+      float gdepth = d_depth ? d_depth[i] : 0.0f;
+      if (flags & kFlagDepth) gdepth += d_feat[i * C + (o++)];
+      if (flags & kFlagNormal) {
+        NormalFwd nf;
+        float nc[3];
+        splat_normal(s, q, x, cen, cam, nf, nc);
+        const float gn[3] = {d_feat[i * C + o], d_feat[i * C + o + 1], d_feat[i * C + o + 2]};
+        splat_normal_vjp(q, cam, nf, gn, gq, gcam);
+      }
+      project_vjp(cam, x, d_uv[2 * i], d_uv[2 * i + 1], gdepth, gx, gcam);
+      float cov[6];
+      covariance3d(s, q, cov);
+      const float g3[3] = {d_conic[3 * i], d_conic[3 * i + 1], d_conic[3 * i + 2]};
+      float gS[6];
+      ewa_vjp(cam, x, cov, W, H, g3, gx, gS, gcam);
+      float gs2[3], gq2[4];
+      covariance3d_vjp(s, q, gS, gs2, gq2);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gs[k] += gs2[k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) gq[k] += gq2[k];
+    } else if (d_depth != nullptr) {
+      // projected but not rasterised: only a direct depth gradient applies
+      float u, v, dd;
+      project(cam, x, nearest, extent, W, H, u, v, dd);
+      if (dd != 0.0f) project_vjp(cam, x, 0.0f, 0.0f, d_depth[i], gx, gcam);
+    }
+    d_xyz[3 * i] = gx.x;
+    d_xyz[3 * i + 1] = gx.y;
+    d_xyz[3 * i + 2] = gx.z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d_rot[4 * i + k] = gq[k];
+    float *ds = d_shs + i * (KS * 3);
+    if constexpr ((KS * 3) % 4 == 0) {
+      float4 *d4 = reinterpret_cast<float4 *>(ds);
+#pragma unroll
+      for (int j = 0; j < KS * 3 / 4; ++j) {
+        float4 t;
+        t.x = (4 * j + 0 < KE * 3) ? dsh[(4 * j + 0) % (KE * 3)] : 0.0f;
+        t.y = (4 * j + 1 < KE * 3) ? dsh[(4 * j + 1) % (KE * 3)] : 0.0f;
+        t.z = (4 * j + 2 < KE * 3) ? dsh[(4 * j + 2) % (KE * 3)] : 0.0f;
+        t.w = (4 * j + 3 < KE * 3) ? dsh[(4 * j + 3) % (KE * 3)] : 0.0f;
+        d4[j] = t;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < KS * 3; ++j) ds[j] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+    }
+  }
+  if (d_intr != nullptr || d_extr != nullptr) block_cam_grad(gcam, d_intr, d_extr);
+  if (d_center != nullptr) {
+    __shared__ float s_c[16][3];
+    __syncthreads();
+    const int wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float sum = wave_sum(gcen[k]);
+      if (lane_id() == 0) s_c[wid][k] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      float sum = 0.0f;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sum += s_c[w][threadIdx.x];
+      if (sum != 0.0f) atomicAdd(d_center + threadIdx.x, sum);
+    }
+  }
+}
+
+template <int KS>
+int launch_fwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const float *scale,
+                  const float *rot, const float *shs, int64_t n, const float *intr,
+                  const float *extr, const float *center, int W, int H, int gx, int gy,
+                  float nearest, float extent, int flags, int C, float *uv, float *depth,
+                  float *cov_out, float *conic, int32_t *radius, int32_t *tiles, float *feat) {
+#define GSR_FWD_ARGS                                                                          \
+  xyz, scale, rot, shs, n, intr, extr, center, W, H, gx, gy, nearest, extent, flags, C, uv, \
+      depth, cov_out, conic, radius, tiles, feat
+  switch (KE) {
+    case 1: preprocess_fwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break;
+    case 4: if constexpr (KS >= 4) { preprocess_fwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break; } return GSR_ERR_ARG;
+    case 9: if constexpr (KS >= 9) { preprocess_fwd_kernel<KS, 9><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break; } return GSR_ERR_ARG;
+    case 16: if constexpr (KS >= 16) { preprocess_fwd_kernel<KS, 16><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break; } return GSR_ERR_ARG;
+    default: return GSR_ERR_ARG;
+  }
+#undef GSR_FWD_ARGS
+  return GSR_OK;
+}
+
+template <int KS>
+int launch_bwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const float *scale,
+                  const float *rot, const float *shs, int64_t n, const float *intr,
+                  const float *extr, const float *center, int W, int H, float nearest,
+                  float extent, int flags, int C, const int32_t *radius, const float *d_uv,
+                  const float *d_depth, const float *d_conic, const float *d_feat, float *d_xyz,
+                  float *d_scale, float *d_rot, float *d_shs, float *d_intr, float *d_extr,
+                  float *d_center) {
+#define GSR_BWD_ARGS                                                                          \
+  xyz, scale, rot, shs, n, intr, extr, center, W, H, nearest, extent, flags, C, radius, d_uv, \
+      d_depth, d_conic, d_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, d_extr, d_center
+  switch (KE) {
+    case 1: preprocess_bwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break;
+    case 4: if constexpr (KS >= 4) { preprocess_bwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
+    case 9: if constexpr (KS >= 9) { preprocess_bwd_kernel<KS, 9><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
+    case 16: if constexpr (KS >= 16) { preprocess_bwd_kernel<KS, 16><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
+    default: return GSR_ERR_ARG;
+  }
+#undef GSR_BWD_ARGS
+  return GSR_OK;
+}
+
+inline int effective_k(int K, int degree) {
+  if (degree < 0) return K;
+  const int k = (degree + 1) * (degree + 1);
+  return k < K ? k : K;
+}
+
+inline int feat_channels(int flags) {
+  return 3 + ((flags & kFlagDepth) ? 1 : 0) + ((flags & kFlagNormal) ? 3 : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
+                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *cam_center, int W, int H, float nearest, float extent,
+                       int flags, float *uv, float *depth, float *cov3d, float *conic,
+                       int32_t *radius, int32_t *tiles, float *feat, void *stream) {
+  GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
+                  (flags & ~3) == 0,
+              "preprocess_fwd: bad arguments (K=%d flags=%d)", K, flags);
+  if (n == 0) return GSR_OK;
+  const int gx = (W + 15) / 16, gy = (H + 15) / 16;
+  const int KE = effective_k(K, degree), C = feat_channels(flags);
+  const unsigned nb = gsr_blocks(n, kThreads);
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+#define GSR_ARGS                                                                              \
+  KE, nb, st, xyz, scale, rot, shs, n, intr, extr, cam_center, W, H, gx, gy, nearest, extent, \
+      flags, C, uv, depth, cov3d, conic, radius, tiles, feat
+  switch (K) {
+    case 1: rc = launch_fwd_ks<1>(GSR_ARGS); break;
+    case 4: rc = launch_fwd_ks<4>(GSR_ARGS); break;
+    case 9: rc = launch_fwd_ks<9>(GSR_ARGS); break;
+    default: rc = launch_fwd_ks<16>(GSR_ARGS); break;
+  }
+#undef GSR_ARGS
+  GSR_REQUIRE(rc == GSR_OK, "preprocess_fwd: unsupported SH size");
+  return gsr_launch_status("preprocess_fwd");
+}
+
+int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
+                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *cam_center, int W, int H, float nearest, float extent,
+                       int flags, const int32_t *radius, const float *d_uv, const float *d_depth,
+                       const float *d_conic, const float *d_feat, float *d_xyz, float *d_scale,
+                       float *d_rot, float *d_shs, float *d_intr, float *d_extr,
+                       float *d_center, void *stream) {
+  GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
+                  (flags & ~3) == 0,
+              "preprocess_bwd: bad arguments (K=%d flags=%d)", K, flags);
+  if (n == 0) return GSR_OK;
+  const int KE = effective_k(K, degree), C = feat_channels(flags);
+  const unsigned nb = gsr_blocks(n, kThreads);
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+#define GSR_ARGS                                                                               \
+  KE, nb, st, xyz, scale, rot, shs, n, intr, extr, cam_center, W, H, nearest, extent, flags, C, \
+      radius, d_uv, d_depth, d_conic, d_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, d_extr,     \
+      d_center
+  switch (K) {
+    case 1: rc = launch_bwd_ks<1>(GSR_ARGS); break;
+    case 4: rc = launch_bwd_ks<4>(GSR_ARGS); break;
+    case 9: rc = launch_bwd_ks<9>(GSR_ARGS); break;
+    default: rc = launch_bwd_ks<16>(GSR_ARGS); break;
+  }
+#undef GSR_ARGS
+  GSR_REQUIRE(rc == GSR_OK, "preprocess_bwd: unsupported SH size");
+  return gsr_launch_status("preprocess_bwd");
+}
+
+}  // extern "C"

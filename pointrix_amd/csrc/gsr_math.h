@@ -414,6 +414,74 @@ __device__ __forceinline__ void sh_basis_vjp(float3 d, int K, const float (&coef
   gd.z += gz;
 }
 
+// ---------------------------------------------------------------------------
+// Per-splat normal channel (examples/supervise/model.py:50-74): the axis of
+// the smallest scale, m = R(q) e_k, normalised, turned towards the camera,
+// rotated into camera space by the extrinsic rotation.
+// ---------------------------------------------------------------------------
+struct NormalFwd {
+  int k;        // index of the smallest scale (first on ties)
+  float sgn;    // +1 or -1 (faces the camera)
+  float m[3];   // R(q) column k
+  float inv;    // 1 / |m|
+  float nh[3];  // sgn * m / |m| (world space)
+};
+
+__device__ __forceinline__ void splat_normal(float3 s, float4 q, float3 x, const float (&center)[3],
+                                             const CamV &cam, NormalFwd &nf, float (&ncam)[3]) {
+  float R[9];
+  quat_rotmat(q, R);
+  int k = 0;
+  float smin = s.x;
+  if (s.y < smin) { k = 1; smin = s.y; }
+  if (s.z < smin) { k = 2; }
+  nf.k = k;
+  nf.m[0] = R[k];
+  nf.m[1] = R[3 + k];
+  nf.m[2] = R[6 + k];
+  const float len = sqrtf(nf.m[0] * nf.m[0] + nf.m[1] * nf.m[1] + nf.m[2] * nf.m[2]);
+  nf.inv = 1.0f / fmaxf(len, 1e-12f);
+  const float dot = (center[0] - x.x) * nf.m[0] + (center[1] - x.y) * nf.m[1] +
+                    (center[2] - x.z) * nf.m[2];
+  nf.sgn = dot < 0.0f ? -1.0f : 1.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nf.nh[i] = nf.sgn * (nf.m[i] * nf.inv);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    ncam[i] = cam.R[3 * i] * nf.nh[0] + cam.R[3 * i + 1] * nf.nh[1] + cam.R[3 * i + 2] * nf.nh[2];
+}
+
+// g (camera-space normal gradient) -> quaternion gradient (+=) and extrinsic
+// rotation gradient gcam[4 + 4i + j] (+=).
+__device__ __forceinline__ void splat_normal_vjp(float4 q, const CamV &cam, const NormalFwd &nf,
+                                                 const float (&g)[3], float (&gq)[4],
+                                                 float (&gcam)[16]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gcam[4 + 4 * i + j] += g[i] * nf.nh[j];
+  float gn[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    gn[j] = nf.sgn * (cam.R[j] * g[0] + cam.R[3 + j] * g[1] + cam.R[6 + j] * g[2]);
+  // through m / |m|
+  const float u0 = nf.m[0] * nf.inv, u1 = nf.m[1] * nf.inv, u2 = nf.m[2] * nf.inv;
+  const float dp = u0 * gn[0] + u1 * gn[1] + u2 * gn[2];
+  float dR[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  dR[nf.k] = (gn[0] - u0 * dp) * nf.inv;
+  dR[3 + nf.k] = (gn[1] - u1 * dp) * nf.inv;
+  dR[6 + nf.k] = (gn[2] - u2 * dp) * nf.inv;
+  const float w = q.x, x = q.y, y = q.z, z = q.w;
+  gq[0] += 2.0f * (w * dR[0] - z * dR[1] + y * dR[2] + z * dR[3] + w * dR[4] - x * dR[5] -
+                   y * dR[6] + x * dR[7] + w * dR[8]);
+  gq[1] += 2.0f * (x * dR[0] + y * dR[1] + z * dR[2] + y * dR[3] - x * dR[4] - w * dR[5] +
+                   z * dR[6] + w * dR[7] - x * dR[8]);
+  gq[2] += 2.0f * (-y * dR[0] + x * dR[1] + w * dR[2] + x * dR[3] + y * dR[4] + z * dR[5] -
+                   w * dR[6] + z * dR[7] - y * dR[8]);
+  gq[3] += 2.0f * (-z * dR[0] - w * dR[1] + x * dR[2] + w * dR[3] - z * dR[4] + y * dR[5] +
+                   x * dR[6] + y * dR[7] + z * dR[8]);
+}
+
 // Camera-gradient block reduction: gcam[0..3] -> d_intr, gcam[4..15] -> d_extr.
 // Must be reached by every thread of the block.
 __device__ __forceinline__ void block_cam_grad(float (&gcam)[16], float *d_intr, float *d_extr) {

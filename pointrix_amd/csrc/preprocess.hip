@@ -166,43 +166,48 @@ __global__ __launch_bounds__(kThreads) void ewa_bwd_kernel(
 // --------------------------------------------------------------------------
 // compute_sh
 // --------------------------------------------------------------------------
+template <int KE>
 __global__ __launch_bounds__(kThreads) void sh_fwd_kernel(
     const float *__restrict__ shs, int64_t s_n, int64_t s_c, int64_t s_k,
-    const float *__restrict__ dirs, int64_t n, int C, int K, float *__restrict__ out) {
+    const float *__restrict__ dirs, int64_t n, int C, float *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float Y[16];
-  sh_basis(make_float3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), K, Y);
+  sh_basis(make_float3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), KE, Y);
   const float *base = shs + i * s_n;
   for (int c = 0; c < C; ++c) {
     const float *sc = base + c * s_c;
     float acc = sc[0] * Y[0];
-    for (int k = 1; k < K; ++k) acc = acc + sc[k * s_k] * Y[k];
+#pragma unroll
+    for (int k = 1; k < KE; ++k) acc = acc + sc[k * s_k] * Y[k];
     out[i * C + c] = acc;
   }
 }
 
+template <int KE>
 __global__ __launch_bounds__(kThreads) void sh_bwd_kernel(
     const float *__restrict__ shs, int64_t s_n, int64_t s_c, int64_t s_k,
-    const float *__restrict__ dirs, int64_t n, int C, int Kfull, int K,
+    const float *__restrict__ dirs, int64_t n, int C, int Kfull,
     const float *__restrict__ d_out, float *__restrict__ d_shs, float *__restrict__ d_dirs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float3 d = make_float3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
   float Y[16];
-  sh_basis(d, K, Y);
+  sh_basis(d, KE, Y);
   float3 gd = make_float3(0.f, 0.f, 0.f);
   const float *base = shs + i * s_n;
   for (int c = 0; c < C; ++c) {
     const float g = d_out[i * C + c];
     float *dst = d_shs + (i * C + c) * Kfull;
-    for (int k = 0; k < K; ++k) dst[k] = g * Y[k];
-    for (int k = K; k < Kfull; ++k) dst[k] = 0.0f;
-    if (d_dirs != nullptr && K > 1) {
+#pragma unroll
+    for (int k = 0; k < KE; ++k) dst[k] = g * Y[k];
+    for (int k = KE; k < Kfull; ++k) dst[k] = 0.0f;
+    if (KE > 1 && d_dirs != nullptr) {
       float coef[16];
       const float *sc = base + c * s_c;
-      for (int k = 0; k < K; ++k) coef[k] = g * sc[k * s_k];
-      sh_basis_vjp(d, K, coef, gd);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) coef[k] = k < KE ? g * sc[(k < KE ? k : 0) * s_k] : 0.0f;
+      sh_basis_vjp(d, KE, coef, gd);
     }
   }
   if (d_dirs != nullptr) {
@@ -288,8 +293,14 @@ int gsr_compute_sh_fwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
               "compute_sh: K must be 1, 4, 9 or 16 (got %d)", K);
   if (n == 0) return GSR_OK;
   const int Ke = degree < 0 ? K : (K < (degree + 1) * (degree + 1) ? K : (degree + 1) * (degree + 1));
-  sh_fwd_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, (hipStream_t)stream>>>(
-      shs, s_n, s_c, s_k, dirs, n, C, Ke, out);
+  const unsigned nb = gsr_blocks(n, kThreads);
+  hipStream_t st = (hipStream_t)stream;
+  switch (Ke) {
+    case 1: sh_fwd_kernel<1><<<nb, kThreads, 0, st>>>(shs, s_n, s_c, s_k, dirs, n, C, out); break;
+    case 4: sh_fwd_kernel<4><<<nb, kThreads, 0, st>>>(shs, s_n, s_c, s_k, dirs, n, C, out); break;
+    case 9: sh_fwd_kernel<9><<<nb, kThreads, 0, st>>>(shs, s_n, s_c, s_k, dirs, n, C, out); break;
+    default: sh_fwd_kernel<16><<<nb, kThreads, 0, st>>>(shs, s_n, s_c, s_k, dirs, n, C, out);
+  }
   return gsr_launch_status("compute_sh_fwd");
 }
 
@@ -300,8 +311,17 @@ int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
               "compute_sh_bwd: K must be 1, 4, 9 or 16 (got %d)", K);
   if (n == 0) return GSR_OK;
   const int Ke = degree < 0 ? K : (K < (degree + 1) * (degree + 1) ? K : (degree + 1) * (degree + 1));
-  sh_bwd_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, (hipStream_t)stream>>>(
-      shs, s_n, s_c, s_k, dirs, n, C, K, Ke, d_out, d_shs, d_dirs);
+  const unsigned nb = gsr_blocks(n, kThreads);
+  hipStream_t st = (hipStream_t)stream;
+#define GSR_SHB(KE_) \
+  sh_bwd_kernel<KE_><<<nb, kThreads, 0, st>>>(shs, s_n, s_c, s_k, dirs, n, C, K, d_out, d_shs, d_dirs)
+  switch (Ke) {
+    case 1: GSR_SHB(1); break;
+    case 4: GSR_SHB(4); break;
+    case 9: GSR_SHB(9); break;
+    default: GSR_SHB(16);
+  }
+#undef GSR_SHB
   return gsr_launch_status("compute_sh_bwd");
 }
 

@@ -105,6 +105,9 @@ class MsplatRender:
         render_depth: bool = False
         render_normal: bool = False
         render_opacity: bool = False
+        # one fused HIP kernel for direction/SH/projection/covariance/EWA
+        # (False: the six msplat-style ops one after another)
+        fused: bool = True
 
     def __init__(self, cfg=None, white_bg=False, device="cuda", **kwargs):
         if cfg is None:
@@ -128,6 +131,10 @@ class MsplatRender:
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
                     position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        if self.cfg.fused:
+            return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
+                                           camera_center, position, opacity, scaling, rotation,
+                                           shs, normals)
         direction = position - camera_center.reshape(1, 3)
         direction = direction / direction.norm(dim=1, keepdim=True)
         # SH warm-up: the first (sh_degree+1)^2 coefficients (msplat.py:98-101)
@@ -164,6 +171,42 @@ class MsplatRender:
         split = Render_Features.split(rendered)
         if "normal" in split:
             # [-1,1] normals to [0,1] (examples/supervise/renderer.py:89-96)
+            nrm = split["normal"]
+            split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        if self.cfg.render_opacity:
+            split["opacity"] = (1.0 - T_final)[None]
+        return {"rendered_features_split": split, "uv_points": ndc,
+                "visibility": radius > 0, "radii": radius}
+
+    def _render_iter_fused(self, height, width, extrinsic_matrix, intrinsic_params,
+                           camera_center, position, opacity, scaling, rotation, shs, normals):
+        normal_in_kernel = normals is None and self.cfg.render_normal
+        uv, depth, conic, radius, tiles, feat, cov3d = ops.preprocess(
+            position, scaling, rotation, shs, intrinsic_params, extrinsic_matrix, camera_center,
+            width, height, degree=self.sh_degree, depth_channel=self.cfg.render_depth,
+            normal_channel=normal_in_kernel, keep_cov=self.keep_intermediates)
+        names = [("rgb", 3)] + ([("depth", 1)] if self.cfg.render_depth else []) + \
+            ([("normal", 3)] if normal_in_kernel else [])
+        if normals is not None:
+            feat = torch.cat([feat, normals.to(feat.dtype)], -1)
+            names.append(("normal", normals.shape[-1]))
+        ids, tile_range = ops.sort_gaussian(uv, depth, width, height, radius, tiles,
+                                            workspace=self.workspace)
+        ndc = torch.zeros_like(uv, requires_grad=True)
+        ndc.retain_grad()
+        rendered, T_final, n_contrib = ops.alpha_blending_ex(uv, conic, opacity, feat, ids,
+                                                             tile_range, self.bg_color, width,
+                                                             height, ndc)
+        if self.keep_intermediates:
+            self.intermediates = {"uv": uv, "depth": depth, "cov3d": cov3d, "conic": conic,
+                                  "radius": radius, "ids": ids, "tile_range": tile_range,
+                                  "features": feat, "rendered": rendered, "T_final": T_final,
+                                  "n_contrib": n_contrib}
+        split, start = {}, 0
+        for name, c in names:
+            split[name] = rendered[start:start + c]
+            start += c
+        if "normal" in split:
             nrm = split["normal"]
             split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
         if self.cfg.render_opacity:
@@ -208,6 +251,10 @@ class MsplatNormalRender(MsplatRender):
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
                     position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        if self.cfg.fused:
+            return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
+                                           camera_center, position, opacity, scaling, rotation,
+                                           shs, normals)
         direction = position - camera_center.reshape(1, 3)
         direction = direction / direction.norm(dim=1, keepdim=True)
         rgb = ops.compute_sh(shs.permute(0, 2, 1), direction)

@@ -53,11 +53,12 @@ def _scene(case):
     return inp, cams[0]
 
 
-def _render_gpu(device, inp, cam, a, seed):
+def _render_gpu(device, inp, cam, a, seed, fused=True):
     camk = {"extrinsic_matrix": cam.extrinsic_matrix, "intrinsic_params": cam.intrinsic_params,
             "camera_center": cam.camera_center}
     r = MsplatRender({"render_depth": a["depth"], "render_normal": a["normal"],
-                      "render_opacity": a["opacity"]}, white_bg=bool(a["bg"]), device="cuda")
+                      "render_opacity": a["opacity"], "fused": fused},
+                     white_bg=bool(a["bg"]), device="cuda")
     r.sh_degree = a["sh_degree"]
     r.keep_intermediates = True
     dl = {k: v.to(device).requires_grad_(True) for k, v in {**inp, **camk}.items()}
@@ -70,11 +71,12 @@ def _render_gpu(device, inp, cam, a, seed):
     return r, res, imgs, weights, dl
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("case", list(CASES))
-def test_render_iter_bitexact_chain(device, case):
+def test_render_iter_bitexact_chain(device, case, fused):
     inp, cam = _scene(case)
     a = CASES[case]
-    r, res, imgs, _, _ = _render_gpu(device, inp, cam, a, seed=1)
+    r, res, imgs, _, _ = _render_gpu(device, inp, cam, a, seed=1, fused=fused)
     im = {k: (v.detach().cpu().numpy() if torch.is_tensor(v) else v)
           for k, v in r.intermediates.items()}
     W, H = cam.width, cam.height
@@ -89,6 +91,14 @@ def test_render_iter_bitexact_chain(device, case):
     assert np.array_equal(im["conic"], con) and np.array_equal(im["radius"], rad)
     ids, rng = r32.sort_gaussian(uv, d, W, H, rad)
     assert np.array_equal(im["ids"], ids) and np.array_equal(im["tile_range"], rng)
+    if fused:
+        flags = (1 if a["depth"] else 0) | (2 if a["normal"] else 0)
+        uvf, df, conf, radf, _, feat = r32.preprocess(
+            inp["position"].numpy(), inp["scaling"].numpy(), inp["rotation"].numpy(),
+            inp["shs"].numpy(), K, E, cam.camera_center.numpy(), W, H, a["sh_degree"], flags)
+        assert np.array_equal(uvf, uv) and np.array_equal(conf, con)
+        assert np.array_equal(radf, rad) and np.array_equal(df, d)
+        assert np.array_equal(im["features"], feat)
     # the per-splat features come from torch ops (SH is checked bit-exact on
     # its own); blend them with the restatement
     out, T, nc = r32.alpha_blending(uv, con, inp["opacity"].numpy(), im["features"], ids, rng,
@@ -98,12 +108,13 @@ def test_render_iter_bitexact_chain(device, case):
     assert np.array_equal(im["rendered"], out)
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("snapped", [True, False])
 @pytest.mark.parametrize("case", list(CASES))
-def test_render_iter_vs_fp64(device, case, snapped):
+def test_render_iter_vs_fp64(device, case, snapped, fused):
     inp, cam = _scene(case)
     a = CASES[case]
-    r, res, imgs, weights, dl = _render_gpu(device, inp, cam, a, seed=1)
+    r, res, imgs, weights, dl = _render_gpu(device, inp, cam, a, seed=1, fused=fused)
     snap = None
     if snapped:
         snap = {"uv": r.intermediates["uv"].detach().cpu(),
@@ -136,7 +147,7 @@ def test_render_iter_vs_fp64(device, case, snapped):
     loss64.backward()
     for k in dl:
         e = rel_err(dl[k].grad, hl[k].grad)
-        print(f"{case} snapped={snapped} d{k}: rel err {e:.3g}")
+        print(f"{case} snapped={snapped} fused={fused} d{k}: rel err {e:.3g}")
         assert e < grad_tol, k
     ndc = res["uv_points"].grad
     assert ndc is not None and tuple(ndc.shape) == (dl["position"].shape[0], 2)
