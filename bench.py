@@ -115,29 +115,35 @@ def algorithmic_bytes(N, K, C, P, T, Nv, M):
 
 
 def cpu_baseline(workload_fn, div, seed=0):
-    """FP64 oracle fwd+bwd on a random 1/div subset of the splats (same camera),
-    scaled to steps/s of the full scene by the subset factor."""
+    """FP64 oracle fwd+bwd on a random 1/div subset of the splats (same camera,
+    rgb + depth + normal, gradients into splats and camera), scaled to steps/s
+    of the full scene by the subset factor."""
     from oracle import torch64 as o64
     scene, cams = workload_fn(seed)
     cam = cams[0]
     n = len(scene)
     g = torch.Generator().manual_seed(seed + 7)
     idx = torch.randperm(n, generator=g)[: max(1, n // div)]
-    inp = {k: v[idx].double().requires_grad_(True)
+    inp = {k: v[idx].detach().double().requires_grad_(True)
            for k, v in scene.render_inputs().items()}
+    E = cam.extrinsic_matrix.double().requires_grad_(True)
+    K = cam.intrinsic_params.double().requires_grad_(True)
+    c = cam.camera_center.double().requires_grad_(True)
     threads = torch.get_num_threads()
     t0 = time.perf_counter()
-    ref = o64.render_iter(cam.height, cam.width, cam.extrinsic_matrix.double(),
-                          cam.intrinsic_params.double(), cam.camera_center.double(),
-                          inp["position"], inp["opacity"], inp["scaling"], inp["rotation"],
-                          inp["shs"], sh_degree=3, render_depth=True)
+    normals = o64.splat_normals(inp["position"], inp["scaling"], inp["rotation"], c, E)
+    ref = o64.render_iter(cam.height, cam.width, E, K, c, inp["position"], inp["opacity"],
+                          inp["scaling"], inp["rotation"], inp["shs"], sh_degree=3,
+                          render_depth=True, normals=normals)
     loss = sum(v.abs().mean() for v in ref["rendered_features_split"].values())
     loss.backward()
     dt = time.perf_counter() - t0
     return {"value": 1.0 / (dt * div), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 PyTorch-CPU oracle, fwd+bwd (rgb+depth) on a random 1/{div} subset "
-                      f"({len(idx)} splats) of the scene, same camera; {dt:.1f} s measured, "
-                      f"scaled x{div}; os.cpu_count()={os.cpu_count()}"}
+            "sample": f"FP64 PyTorch-CPU oracle (oracle/torch64.py), fwd+bwd of rgb+depth+normal "
+                      f"with splat and camera gradients, on a random 1/{div} subset "
+                      f"({len(idx)} splats) of the scene with the same camera; {dt:.1f} s "
+                      f"measured, scaled by {div}; torch threads={threads}, "
+                      f"os.cpu_count()={os.cpu_count()}"}
 
 
 def main():
@@ -147,7 +153,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="full3m")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-div", type=int, default=64)
+    ap.add_argument("--cpu-div", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
