@@ -93,14 +93,17 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, float *uv, float *depth, float *cov3d, float *conic,
                        int32_t *radius, int32_t *tiles, float *feat, void *stream);
-/* Backward: d_uv[N,2], d_depth[N] (may be NULL), d_conic[N,3], d_feat[N,C]
- * -> d_xyz[N,3], d_scale[N,3], d_rot[N,4], d_shs[N,K,3] (all overwritten),
- * d_intr[4], d_extr[12], d_center[3] (accumulated; each may be NULL). */
+/* Backward: d_uv[N,2], d_depth[N] (may be NULL), d_conic[N,3], d_feat[N,C],
+ * each addressed with its row stride (ld_*; e.g. views into the d_splat
+ * records of gsr_alpha_blending_bwd) -> d_xyz[N,3], d_scale[N,3], d_rot[N,4],
+ * d_shs[N,K,3] (all overwritten), d_intr[4], d_extr[12], d_center[3]
+ * (accumulated; each may be NULL). */
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
-                       int flags, const int32_t *radius, const float *d_uv, const float *d_depth,
-                       const float *d_conic, const float *d_feat, float *d_xyz, float *d_scale,
+                       int flags, const int32_t *radius, const float *d_uv, int ld_uv,
+                       const float *d_depth, const float *d_conic, int ld_conic,
+                       const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_intr, float *d_extr,
                        float *d_center, void *stream);
 
@@ -132,15 +135,16 @@ int gsr_alpha_blending_fwd(const float *uv, const float *conic, const float *opa
                            const int32_t *tile_range, float bg, int W, int H, int tile_w,
                            int tile_h, float *out, float *T_final, int32_t *n_contrib,
                            void *stream);
-/* Backward: d_out[C,H,W], d_T_final[H,W] (may be NULL) -> gradients,
- * accumulated (+=) into d_uv[N,2], d_conic[N,3], d_opacity[N], d_feature[N,C]
- * (caller zero-fills). */
+/* Backward: d_out[C,H,W], d_T_final[H,W] (may be NULL) -> per-splat gradient
+ * records accumulated (+=) into d_splat[N, ld] (ld >= 6 + C; caller
+ * zero-fills): [d_u, d_v, d_conic_a, d_conic_b, d_conic_c, d_opacity,
+ * d_feature_0 .. d_feature_{C-1}]. */
 int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opacity,
                            const float *feature, int C, const int32_t *ids,
                            const int32_t *tile_range, float bg, int W, int H, int tile_w,
                            int tile_h, const float *T_final, const int32_t *n_contrib,
-                           const float *d_out, const float *d_T_final, float *d_uv,
-                           float *d_conic, float *d_opacity, float *d_feature, void *stream);
+                           const float *d_out, const float *d_T_final, float *d_splat, int ld,
+                           void *stream);
 
 #ifdef __cplusplus
 }

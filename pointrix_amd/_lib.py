@@ -35,7 +35,7 @@ SIGNATURES = {
     "gsr_preprocess_fwd": (_I, [_P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I, _P,
                                 _P, _P, _P, _P, _P, _P, _P]),
     "gsr_preprocess_bwd": (_I, [_P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I, _P,
-                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+                                _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_sort_ws_gauss_bytes": (_SZ, [_I64]),
     "gsr_sort_ws_inst_bytes": (_SZ, [_I64]),
     "gsr_sort_count": (_I, [_P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),
@@ -43,7 +43,7 @@ SIGNATURES = {
     "gsr_alpha_blending_fwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,
                                     _P]),
     "gsr_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,
-                                    _P, _P, _P, _P, _P, _P]),
+                                    _P, _P, _I, _P]),
 }
 
 _lock = threading.Lock()

@@ -9,8 +9,10 @@
 //
 // Backward: each pixel starts from its saved T_final and contributor count and
 // walks its splats back to front, recovering T_i = T_{i+1} / (1 - alpha_i)
-// instead of storing it.  Per-splat gradients are summed across the wave64
-// before one atomic per wave and value.
+// instead of storing it.  The 6 + C per-splat gradients of the wave's 64
+// pixels are summed with a butterfly transpose reduction, after which one
+// atomic instruction (one lane per value) adds them into the splat's
+// contiguous gradient record.
 #include "gsr_internal.h"
 
 using namespace gsr;
@@ -108,15 +110,25 @@ __global__ __launch_bounds__(TW *TH) void blend_fwd_kernel(
   }
 }
 
+// Per-splat gradient record written by the backward: [du, dv, da, db, dc,
+// dopacity, dfeature_0 .. dfeature_{C-1}] at d_splat[g * ld + ...].
+template <int C>
+struct GradLayout {
+  static constexpr int Q = 6 + C;
+  static constexpr int QP = Q <= 8 ? 8 : (Q <= 16 ? 16 : 32);  // butterfly width
+};
+
 template <int C, int TW, int TH>
 __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
     const float *__restrict__ uv, const float *__restrict__ conic,
     const float *__restrict__ opacity, const float *__restrict__ feat,
     const int32_t *__restrict__ ids, const int32_t *__restrict__ ranges, float bg, int W, int H,
     int gx, const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
-    const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_uv,
-    float *__restrict__ d_conic, float *__restrict__ d_op, float *__restrict__ d_feat) {
+    const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
+    int ld) {
   constexpr int B = TW * TH;
+  constexpr int QP = GradLayout<C>::QP;
+  constexpr int Q = GradLayout<C>::Q;
   __shared__ SplatLds s_sp[B];
   __shared__ float s_f[C * B];
   __shared__ int s_id[B];
@@ -130,6 +142,9 @@ __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
   const int start = ranges[2 * tile], end = ranges[2 * tile + 1];
   const int64_t pix = (int64_t)py * W + px;
   const int64_t HW = (int64_t)H * W;
+  const int lane = lane_id();
+  const int my_q = bfly_index<QP>(lane);
+  const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
 
   float T = 1.0f, gpix[C];
   int last = 0;
@@ -149,7 +164,7 @@ __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
   if (last > 0) atomicMax(&s_max, last);
   __syncthreads();
   const int stop = start + s_max;
-  if (stop <= start) return;  // uniform
+  if (stop <= start || end <= start) return;  // uniform
   const float T_final = T;
   float gsum = 0.0f;
 #pragma unroll
@@ -178,9 +193,9 @@ __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
     __syncthreads();
     for (int k = bend - bstart - 1; k >= 0; --k) {
       const int local = bstart + k - start;
-      float gu = 0.f, gv = 0.f, ga = 0.f, gb = 0.f, gc = 0.f, go = 0.f, gf[C];
+      float v[QP];
 #pragma unroll
-      for (int c = 0; c < C; ++c) gf[c] = 0.0f;
+      for (int q = 0; q < QP; ++q) v[q] = 0.0f;
       bool contrib = local < last;
       if (contrib) {
         const SplatLds sp = s_sp[k];
@@ -189,52 +204,35 @@ __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
         const float alpha = splat_alpha(sp.co, splat_power(sp.co, dx, dy), G);
         contrib = alpha >= 0.0f;
         if (contrib) {
-          const float one_m = 1.0f - alpha;
-          T = T / one_m;
+          const float inv_om = __builtin_amdgcn_rcpf(1.0f - alpha);
+          T = T * inv_om;
           const float w = alpha * T;
           float dL_dalpha = 0.0f;
 #pragma unroll
           for (int c = 0; c < C; ++c) {
             const float f = s_f[c * B + k];
-            gf[c] = w * gpix[c];
+            v[6 + c] = w * gpix[c];
             accum[c] = last_alpha * last_f[c] + (1.0f - last_alpha) * accum[c];
             last_f[c] = f;
             dL_dalpha += (f - accum[c]) * gpix[c];
           }
           dL_dalpha *= T;
-          dL_dalpha += (-T_final / one_m) * bg_term;
+          dL_dalpha -= T_final * inv_om * bg_term;
           last_alpha = alpha;
           if (sp.co.w * G < kAlphaMax) {  // alpha not clamped
-            go = G * dL_dalpha;
+            v[5] = G * dL_dalpha;
             const float dpow = alpha * dL_dalpha;
-            gu = dpow * (2.0f * sp.co.x * dx + sp.co.y * dy);
-            gv = dpow * (sp.co.y * dx + 2.0f * sp.co.z * dy);
-            ga = dpow * (-0.5f * dx * dx);
-            gb = dpow * (-dx * dy);
-            gc = dpow * (-0.5f * dy * dy);
+            v[0] = dpow * (2.0f * sp.co.x * dx + sp.co.y * dy);
+            v[1] = dpow * (sp.co.y * dx + 2.0f * sp.co.z * dy);
+            v[2] = dpow * (-0.5f * dx * dx);
+            v[3] = dpow * (-dx * dy);
+            v[4] = dpow * (-0.5f * dy * dy);
           }
         }
       }
       if (__ballot(contrib) == 0ull) continue;  // wave-uniform
-      gu = wave_sum(gu);
-      gv = wave_sum(gv);
-      ga = wave_sum(ga);
-      gb = wave_sum(gb);
-      gc = wave_sum(gc);
-      go = wave_sum(go);
-#pragma unroll
-      for (int c = 0; c < C; ++c) gf[c] = wave_sum(gf[c]);
-      if (lane_id() == 0) {
-        const int g = s_id[k];
-        atomicAdd(d_uv + 2 * g, gu);
-        atomicAdd(d_uv + 2 * g + 1, gv);
-        atomicAdd(d_conic + 3 * g, ga);
-        atomicAdd(d_conic + 3 * g + 1, gb);
-        atomicAdd(d_conic + 3 * g + 2, gc);
-        atomicAdd(d_op + g, go);
-#pragma unroll
-        for (int c = 0; c < C; ++c) atomicAdd(d_feat + (int64_t)g * C + c, gf[c]);
-      }
+      Bfly<QP, 32>::run(v, lane);
+      if (writer) atomicAdd(d_splat + (int64_t)s_id[k] * ld + my_q, v[0]);
     }
   }
 }
@@ -263,13 +261,12 @@ template <int C>
 int launch_bwd(const float *uv, const float *conic, const float *opacity, const float *feature,
                const int32_t *ids, const int32_t *tile_range, float bg, int W, int H, int tw,
                int th, const float *T_final, const int32_t *n_contrib, const float *d_out,
-               const float *d_T_final, float *d_uv, float *d_conic, float *d_opacity,
-               float *d_feature, hipStream_t s) {
+               const float *d_T_final, float *d_splat, int ld, hipStream_t s) {
   const int gx = (W + tw - 1) / tw, gy = (H + th - 1) / th;
   const unsigned nt = (unsigned)(gx * gy);
 #define GSR_BWD_ARGS                                                                          \
   uv, conic, opacity, feature, ids, tile_range, bg, W, H, gx, T_final, n_contrib, d_out, \
-      d_T_final, d_uv, d_conic, d_opacity, d_feature
+      d_T_final, d_splat, ld
   if (tw == 16 && th == 16)
     blend_bwd_kernel<C, 16, 16><<<nt, 256, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 8 && th == 32)
@@ -325,14 +322,14 @@ int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opa
                            const float *feature, int C, const int32_t *ids,
                            const int32_t *tile_range, float bg, int W, int H, int tile_w,
                            int tile_h, const float *T_final, const int32_t *n_contrib,
-                           const float *d_out, const float *d_T_final, float *d_uv,
-                           float *d_conic, float *d_opacity, float *d_feature, void *stream) {
-  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16, "alpha_blending_bwd: bad sizes (C=%d)", C);
+                           const float *d_out, const float *d_T_final, float *d_splat, int ld,
+                           void *stream) {
+  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16 && ld >= 6 + C,
+              "alpha_blending_bwd: bad sizes (C=%d, ld=%d)", C, ld);
   int rc = GSR_OK;
   GSR_DISPATCH_C(C, rc = launch_bwd<CC>(uv, conic, opacity, feature, ids, tile_range, bg, W, H,
                                         tile_w, tile_h, T_final, n_contrib, d_out, d_T_final,
-                                        d_uv, d_conic, d_opacity, d_feature,
-                                        (hipStream_t)stream));
+                                        d_splat, ld, (hipStream_t)stream));
   GSR_REQUIRE(rc == GSR_OK, "alpha_blending_bwd: unsupported tile shape %dx%d", tile_w, tile_h);
   return gsr_launch_status("alpha_blending_bwd");
 }

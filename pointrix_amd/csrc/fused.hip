@@ -134,9 +134,9 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     const float *__restrict__ rot, const float *__restrict__ shs, int64_t n,
     const float *__restrict__ intr, const float *__restrict__ extr,
     const float *__restrict__ center, int W, int H, float nearest, float extent, int flags,
-    int C, const int32_t *__restrict__ radius, const float *__restrict__ d_uv,
-    const float *__restrict__ d_depth, const float *__restrict__ d_conic,
-    const float *__restrict__ d_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
+    int C, const int32_t *__restrict__ radius, const float *__restrict__ d_uv, int ld_uv,
+    const float *__restrict__ d_depth, const float *__restrict__ d_conic, int ld_conic,
+    const float *__restrict__ d_feat, int ld_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
     float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_intr,
     float *__restrict__ d_extr, float *__restrict__ d_center) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         float acc = sh[c] * Y[0];
 #pragma unroll
         for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
-        const float g = (acc + 0.5f >= 0.0f) ? d_feat[i * C + c] : 0.0f;
+        const float g = (acc + 0.5f >= 0.0f) ? d_feat[i * ld_feat + c] : 0.0f;
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
           dsh[3 * k + c] = g * Y[k];
@@ -194,18 +194,20 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       }
       int o = 3;
       float gdepth = d_depth ? d_depth[i] : 0.0f;
-      if (flags & kFlagDepth) gdepth += d_feat[i * C + (o++)];
+      if (flags & kFlagDepth) gdepth += d_feat[i * ld_feat + (o++)];
       if (flags & kFlagNormal) {
         NormalFwd nf;
         float nc[3];
         splat_normal(s, q, x, cen, cam, nf, nc);
-        const float gn[3] = {d_feat[i * C + o], d_feat[i * C + o + 1], d_feat[i * C + o + 2]};
+        const float gn[3] = {d_feat[i * ld_feat + o], d_feat[i * ld_feat + o + 1],
+                             d_feat[i * ld_feat + o + 2]};
         splat_normal_vjp(q, cam, nf, gn, gq, gcam);
       }
-      project_vjp(cam, x, d_uv[2 * i], d_uv[2 * i + 1], gdepth, gx, gcam);
+      project_vjp(cam, x, d_uv[i * ld_uv], d_uv[i * ld_uv + 1], gdepth, gx, gcam);
       float cov[6];
       covariance3d(s, q, cov);
-      const float g3[3] = {d_conic[3 * i], d_conic[3 * i + 1], d_conic[3 * i + 2]};
+      const float g3[3] = {d_conic[i * ld_conic], d_conic[i * ld_conic + 1],
+                           d_conic[i * ld_conic + 2]};
       float gS[6];
       ewa_vjp(cam, x, cov, W, H, g3, gx, gS, gcam);
       float gs2[3], gq2[4];
@@ -288,12 +290,14 @@ int launch_bwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const f
                   const float *rot, const float *shs, int64_t n, const float *intr,
                   const float *extr, const float *center, int W, int H, float nearest,
                   float extent, int flags, int C, const int32_t *radius, const float *d_uv,
-                  const float *d_depth, const float *d_conic, const float *d_feat, float *d_xyz,
+                  int ld_uv, const float *d_depth, const float *d_conic, int ld_conic,
+                  const float *d_feat, int ld_feat, float *d_xyz,
                   float *d_scale, float *d_rot, float *d_shs, float *d_intr, float *d_extr,
                   float *d_center) {
 #define GSR_BWD_ARGS                                                                          \
   xyz, scale, rot, shs, n, intr, extr, center, W, H, nearest, extent, flags, C, radius, d_uv, \
-      d_depth, d_conic, d_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, d_extr, d_center
+      ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, \
+      d_extr, d_center
   switch (KE) {
     case 1: preprocess_bwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_bwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
@@ -350,8 +354,9 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
-                       int flags, const int32_t *radius, const float *d_uv, const float *d_depth,
-                       const float *d_conic, const float *d_feat, float *d_xyz, float *d_scale,
+                       int flags, const int32_t *radius, const float *d_uv, int ld_uv,
+                       const float *d_depth, const float *d_conic, int ld_conic,
+                       const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_intr, float *d_extr,
                        float *d_center, void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
@@ -364,8 +369,8 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
   int rc;
 #define GSR_ARGS                                                                               \
   KE, nb, st, xyz, scale, rot, shs, n, intr, extr, cam_center, W, H, nearest, extent, flags, C, \
-      radius, d_uv, d_depth, d_conic, d_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, d_extr,     \
-      d_center
+      radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot,   \
+      d_shs, d_intr, d_extr, d_center
   switch (K) {
     case 1: rc = launch_bwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_bwd_ks<4>(GSR_ARGS); break;

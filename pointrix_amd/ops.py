@@ -313,19 +313,25 @@ class _AlphaBlending(torch.autograd.Function):
         dev = u.device
         g_out = torch.zeros(C, H, W, device=dev) if g_out is None else _f32(g_out)
         g_T = None if g_T is None else _f32(g_T)
-        d_u = torch.zeros(n, 2, device=dev)
-        d_c = torch.zeros(n, 3, device=dev)
-        d_o = torch.zeros(n, device=dev)
-        d_f = torch.zeros(n, C, device=dev)
+        ld = grad_record_width(C)
+        rec = torch.zeros(n, ld, device=dev)
         with _span("alpha_blending_bwd", dev):
             call("gsr_alpha_blending_bwd", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids), ptr(tr),
-                 bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(d_u), ptr(d_c),
-                 ptr(d_o), ptr(d_f), stream_of(u))
+                 bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(rec), ld,
+                 stream_of(u))
+        d_u, d_c, d_o, d_f = rec[:, 0:2], rec[:, 2:5], rec[:, 5], rec[:, 6:6 + C]
         d_ndc = None
         if ctx.needs_input_grad[9]:
             d_ndc = d_u * torch.tensor([0.5 * W, 0.5 * H], device=dev)
         return (d_u, d_c, d_o.reshape(op_shape), d_f, None, None, None, None, None, d_ndc, None,
                 None)
+
+
+def grad_record_width(C: int) -> int:
+    """Row width of the per-splat gradient records of the blend backward:
+    [du, dv, da, db, dc, dopacity, dfeature...] padded to 8, 16 or 32."""
+    q = 6 + C
+    return 8 if q <= 8 else (16 if q <= 16 else 32)
 
 
 def alpha_blending_ex(uv, conic, opacity, feature, ids, tile_range, bg, W, H, ndc=None,
@@ -395,6 +401,15 @@ FLAG_DEPTH = 1
 FLAG_NORMAL = 2
 
 
+def _rows(t: torch.Tensor) -> torch.Tensor:
+    """fp32 2-D tensor whose rows are contiguous (row stride may be larger
+    than the row, e.g. a column slice of the blend's gradient records)."""
+    t = t.detach()
+    if t.dtype != torch.float32 or t.dim() != 2 or t.stride(1) != 1:
+        t = t.to(torch.float32).contiguous()
+    return t
+
+
 def _aligned16(t: torch.Tensor) -> torch.Tensor:
     return t if t.data_ptr() % 16 == 0 else t.clone()
 
@@ -438,9 +453,9 @@ class _Preprocess(torch.autograd.Function):
         n, K = x.shape[0], sh.shape[1]
         dev = x.device
         z = lambda *shape: torch.zeros(*shape, device=dev)
-        g_uv = z(n, 2) if g_uv is None else _f32(g_uv)
-        g_conic = z(n, 3) if g_conic is None else _f32(g_conic)
-        g_feat = z(n, C) if g_feat is None else _f32(g_feat)
+        g_uv = z(n, 2) if g_uv is None else _rows(g_uv)
+        g_conic = z(n, 3) if g_conic is None else _rows(g_conic)
+        g_feat = z(n, C) if g_feat is None else _rows(g_feat)
         g_depth = None if g_depth is None else _f32(g_depth).reshape(-1)
         d_x, d_s, d_q = torch.empty_like(x), torch.empty_like(s), torch.empty_like(q)
         d_sh = torch.empty_like(sh)
@@ -452,8 +467,9 @@ class _Preprocess(torch.autograd.Function):
         with _span("preprocess_bwd", dev):
             call("gsr_preprocess_bwd", ptr(x), ptr(s), ptr(q), ptr(sh), n, K, degree, ptr(k),
                  ptr(e), ptr(c), W, H, nearest, extent, flags, ptr(radius), ptr(g_uv),
-                 ptr(g_depth), ptr(g_conic), ptr(g_feat), ptr(d_x), ptr(d_s), ptr(d_q),
-                 ptr(d_sh), ptr(d_k), ptr(d_e), ptr(d_c), stream_of(x))
+                 g_uv.stride(0), ptr(g_depth), ptr(g_conic), g_conic.stride(0), ptr(g_feat),
+                 g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_k), ptr(d_e),
+                 ptr(d_c), stream_of(x))
         return (d_x, d_s, d_q, d_sh, _intr_grad(d_k, intr) if need_k else None,
                 _extr_grad(d_e, extr) if need_e else None,
                 d_c.reshape(center.shape).to(center.dtype) if need_c else None,

@@ -4,8 +4,9 @@ from the pinned FP64 oracle).
 CPU: the oracle still reproduces them (and the seeded inputs still hash the
 same).  GPU: the HIP renderer matches them without running the oracle:
 images within 1e-5 on >= 99.9% of pixels, radii exact, contributor counts
-equal on >= 99.9% of pixels, gradients within 1e-3 of max|g| (end-to-end FP32
-vs FP64, pixel-coordinate rounding included).
+equal on >= 99.9% of pixels, gradients (end-to-end FP32
+vs FP64, pixel-coordinate rounding included): >= 99.9% of the entries within
+1e-4 of max|g| and relative L2 error below 1e-3.
 """
 import os
 
@@ -65,9 +66,14 @@ def test_gpu_matches_sh3_fixture(device, fused):
         scale = max(1.0, float(np.abs(ref["img_" + k]).max())) if k == "depth" else 1.0
         assert (perr <= 1e-5 * scale).mean() >= 0.999, k
     for k, v in dl.items():
-        g = ref["grad_" + k]
-        err = np.abs(v.grad.cpu().numpy() - g).max() / max(np.abs(g).max(), 1e-30)
-        assert err < 1e-3, (k, err)
+        g = ref["grad_" + k].astype(np.float64)
+        d = np.abs(v.grad.cpu().numpy().astype(np.float64) - g)
+        scale = max(np.abs(g).max(), 1e-30)
+        frac = (d <= 1e-4 * scale).mean()
+        l2 = np.linalg.norm(d) / max(np.linalg.norm(g), 1e-30)
+        print(f"golden d{k}: max {d.max() / scale:.3g}, within 1e-4: {100 * frac:.3f}%, "
+              f"rel L2 {l2:.3g}")
+        assert frac >= 0.999 and l2 < 1e-3, (k, frac, l2)
 
 
 @pytest.mark.gpu
