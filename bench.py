@@ -84,7 +84,7 @@ def make_step(renderer, scene, cam, targets):
     params = scene.params() + cam.params()
 
     def step(stats=None):
-        inp = scene.render_inputs()
+        inp = scene.render_inputs(split_sh=True)
         res = renderer.render_iter(**cam.render_kwargs(), **inp)
         f = res["rendered_features_split"]
         loss = l1(f["rgb"], targets["rgb"])
@@ -200,7 +200,7 @@ def main():
 
     # sizes for the algorithmic-byte model
     with torch.no_grad():
-        res = renderer.render_iter(**cam.render_kwargs(), **scene.render_inputs())
+        res = renderer.render_iter(**cam.render_kwargs(), **scene.render_inputs(split_sh=True))
     Nv = int(res["visibility"].sum().item())
     M = ops.last_instance_count
     N = len(scene)

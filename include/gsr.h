@@ -82,30 +82,33 @@ int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
  * One pass per splat replacing the view-direction normalisation (:94-96),
  * the SH warm-up mask + compute_sh + "+0.5, clamp" (:98-105), project_point,
  * compute_cov3d, ewa_project (:107-130) and RenderFeatures.combine (:137-139).
- * shs: [N,K,3] (K in 1,4,9,16), summed over the first min(K,(degree+1)^2)
- * coefficients (degree < 0: all).  flags: 1 = depth channel, 2 = camera-space
+ * shs: [N,K,3] (K in 1,4,9,16), or -- when shs_rest != NULL -- the DC
+ * coefficients shs[N,1,3] and shs_rest[N,K-1,3] (pointrix's two SH
+ * parameters, gaussian_points.py:86-90, read without concatenation); summed
+ * over the first min(K,(degree+1)^2) coefficients (degree < 0: all).  flags: 1 = depth channel, 2 = camera-space
  * normal channel (shortest splat axis facing the camera,
  * examples/supervise/model.py:50-74).  feat[N,C], C = 3 (+1) (+3):
  * rgb | depth | normal.  uv/depth/conic/radius/tiles equal those of the
  * separate ops; cov3d[N,6] may be NULL. */
 int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
-                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, float *uv, float *depth, float *cov3d, float *conic,
                        int32_t *radius, int32_t *tiles, float *feat, void *stream);
 /* Backward: d_uv[N,2], d_depth[N] (may be NULL), d_conic[N,3], d_feat[N,C],
  * each addressed with its row stride (ld_*; e.g. views into the d_splat
  * records of gsr_alpha_blending_bwd) -> d_xyz[N,3], d_scale[N,3], d_rot[N,4],
- * d_shs[N,K,3] (all overwritten), d_intr[4], d_extr[12], d_center[3]
- * (accumulated; each may be NULL). */
+ * d_shs[N,K,3] (or d_shs[N,1,3] + d_shs_rest[N,K-1,3] when split; all
+ * overwritten), d_intr[4], d_extr[12], d_center[3] (accumulated; each may be
+ * NULL). */
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
-                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, const int32_t *radius, const float *d_uv, int ld_uv,
                        const float *d_depth, const float *d_conic, int ld_conic,
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
-                       float *d_rot, float *d_shs, float *d_intr, float *d_extr,
-                       float *d_center, void *stream);
+                       float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
+                       float *d_extr, float *d_center, void *stream);
 
 /* ---- msplat.sort_gaussian (msplat.py:133-135) ----------------------------
  * Two phases around one host read of M (the instance count):

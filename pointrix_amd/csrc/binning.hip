@@ -280,22 +280,49 @@ struct LoadSortedCount {
   __device__ uint32_t operator()(uint32_t s) const { return cnt[ids[s]]; }
 };
 
+// Emission, wave-cooperative: lane l of a wave loads splat (wave base + l);
+// the wave then walks its 64 splats in order and, for each, all 64 lanes
+// write consecutive instances (coalesced stores, no per-splat imbalance).
+// Row/column of instance k within the w-wide rectangle: q = trunc((k + 0.5) /
+// w) in float is exact while k < 2^22 / w (k is never within 0.5/w of a
+// multiple of w), far beyond the largest tile grid.
 __global__ __launch_bounds__(kThreads) void emit_kernel(
     const float *__restrict__ uv, const int32_t *__restrict__ radius,
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
     const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t *__restrict__ tkeys,
     int32_t *__restrict__ tvals) {
-  const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
-  if (s >= count_of(nv_ptr, cap)) return;
-  const int32_t id = sorted_ids[s];
-  const Rect q = tile_rect(uv[2 * id], uv[2 * id + 1], radius[id], g.gx, g.gy, g.tw, g.th);
-  uint32_t o = offsets[s];
-  for (int ty = q.y0; ty < q.y1; ++ty)
-    for (int tx = q.x0; tx < q.x1; ++tx) {
-      tkeys[o] = (uint32_t)(ty * g.gx + tx);
-      tvals[o] = id;
-      ++o;
+  const uint32_t nv = count_of(nv_ptr, cap);
+  const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
+  if (wave_base >= nv) return;  // uniform per wave
+  const int lane = lane_id();
+  const uint32_t s = wave_base + lane;
+  int id = 0, x0 = 0, y0 = 0, w = 1, cnt = 0;
+  uint32_t off = 0;
+  if (s < nv) {
+    id = sorted_ids[s];
+    const Rect q = tile_rect(uv[2 * id], uv[2 * id + 1], radius[id], g.gx, g.gy, g.tw, g.th);
+    x0 = q.x0;
+    y0 = q.y0;
+    w = max(q.x1 - q.x0, 1);
+    cnt = (q.x1 - q.x0) * (q.y1 - q.y0);
+    off = offsets[s];
+  }
+  const int nsplat = min(64, (int)(nv - wave_base));
+  for (int j = 0; j < nsplat; ++j) {
+    const int jid = __builtin_amdgcn_readlane(id, j);
+    const int jx0 = __builtin_amdgcn_readlane(x0, j);
+    const int jy0 = __builtin_amdgcn_readlane(y0, j);
+    const int jw = __builtin_amdgcn_readlane(w, j);
+    const int jcnt = __builtin_amdgcn_readlane(cnt, j);
+    const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
+    const float inv_w = 1.0f / (float)jw;
+    for (int k = lane; k < jcnt; k += 64) {
+      const int row = (int)(((float)k + 0.5f) * inv_w);
+      const int col = k - row * jw;
+      tkeys[joff + k] = (uint32_t)((jy0 + row) * g.gx + (jx0 + col));
+      tvals[joff + k] = jid;
     }
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void fill_kernel(int32_t *__restrict__ p, int64_t n,

@@ -7,8 +7,8 @@
 // msplat ops, so uv / depth / cov3d / conic / radius are identical to theirs.
 //
 // Layouts: shs is the [N, KS, 3] array render_iter receives (coefficient
-// major, channel minor); only the first KE = min(KS, (degree+1)^2)
-// coefficients are read.  feat is [N, C] with C = 3 (+1 depth) (+3 normal).
+// major, channel minor), or the pair DC [N, 1, 3] + rest [N, KS-1, 3]; only
+// the first KE = min(KS, (degree+1)^2) coefficients are read.  feat is [N, C] with C = 3 (+1 depth) (+3 normal).
 #include "gsr_internal.h"
 #include "gsr_math.h"
 
@@ -20,10 +20,23 @@ constexpr int kThreads = 256;
 constexpr int kFlagDepth = 1;
 constexpr int kFlagNormal = 2;
 
-// Load the first KE coefficients (x3 channels) of splat i.
+// Load the first KE coefficients (x3 channels) of splat i, either from one
+// [N, KS, 3] array or, when rest != nullptr, from the DC array [N, 1, 3] and
+// the higher-order array [N, KS-1, 3] (pointrix keeps them as two
+// parameters, gaussian_points.py:86-90; reading both avoids concatenating).
 template <int KS, int KE>
-__device__ __forceinline__ void load_sh(const float *__restrict__ shs, int64_t i,
+__device__ __forceinline__ void load_sh(const float *__restrict__ shs,
+                                        const float *__restrict__ rest, int64_t i,
                                         float (&sh)[KE * 3]) {
+  if (rest != nullptr) {
+    sh[0] = shs[3 * i];
+    sh[1] = shs[3 * i + 1];
+    sh[2] = shs[3 * i + 2];
+    const float *rb = rest + i * ((KS - 1) * 3);
+#pragma unroll
+    for (int j = 3; j < KE * 3; ++j) sh[j] = rb[j - 3];
+    return;
+  }
   const float *base = shs + i * (KS * 3);
   if constexpr ((KS * 3) % 4 == 0) {
     constexpr int n4 = (KE * 3 + 3) / 4;
@@ -51,9 +64,10 @@ __device__ __forceinline__ float3 view_dir(float3 x, const float (&c)[3], float 
 template <int KS, int KE>
 __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
     const float *__restrict__ xyz, const float *__restrict__ scale,
-    const float *__restrict__ rot, const float *__restrict__ shs, int64_t n,
-    const float *__restrict__ intr, const float *__restrict__ extr,
-    const float *__restrict__ center, int W, int H, int gx, int gy, float nearest,
+    const float *__restrict__ rot, const float *__restrict__ shs,
+    const float *__restrict__ shs_rest, int64_t n, const float *__restrict__ intr,
+    const float *__restrict__ extr, const float *__restrict__ center, int W, int H, int gx, int gy,
+    float nearest,
     float extent, int flags, int C, float *__restrict__ uv, float *__restrict__ depth,
     float *__restrict__ cov_out, float *__restrict__ conic, int32_t *__restrict__ radius,
     int32_t *__restrict__ tiles, float *__restrict__ feat) {
@@ -96,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
       float Y[16];
       sh_basis(dir, KE, Y);
       float sh[KE * 3];
-      load_sh<KS, KE>(shs, i, sh);
+      load_sh<KS, KE>(shs, shs_rest, i, sh);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         float acc = sh[c] * Y[0];
@@ -131,14 +145,15 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
 template <int KS, int KE>
 __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     const float *__restrict__ xyz, const float *__restrict__ scale,
-    const float *__restrict__ rot, const float *__restrict__ shs, int64_t n,
-    const float *__restrict__ intr, const float *__restrict__ extr,
-    const float *__restrict__ center, int W, int H, float nearest, float extent, int flags,
+    const float *__restrict__ rot, const float *__restrict__ shs,
+    const float *__restrict__ shs_rest, int64_t n, const float *__restrict__ intr,
+    const float *__restrict__ extr, const float *__restrict__ center, int W, int H,
+    float nearest, float extent, int flags,
     int C, const int32_t *__restrict__ radius, const float *__restrict__ d_uv, int ld_uv,
     const float *__restrict__ d_depth, const float *__restrict__ d_conic, int ld_conic,
     const float *__restrict__ d_feat, int ld_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
-    float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_intr,
-    float *__restrict__ d_extr, float *__restrict__ d_center) {
+    float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_shs_rest,
+    float *__restrict__ d_intr, float *__restrict__ d_extr, float *__restrict__ d_center) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float gcam[16];
 #pragma unroll
@@ -162,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       float Y[16];
       sh_basis(dir, KE, Y);
       float sh[KE * 3];
-      load_sh<KS, KE>(shs, i, sh);
+      load_sh<KS, KE>(shs, shs_rest, i, sh);
       float coef[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) coef[k] = 0.0f;
@@ -229,21 +244,30 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
 #pragma unroll
     for (int k = 0; k < 4; ++k) d_rot[4 * i + k] = gq[k];
-    float *ds = d_shs + i * (KS * 3);
-    if constexpr ((KS * 3) % 4 == 0) {
-      float4 *d4 = reinterpret_cast<float4 *>(ds);
+    if (d_shs_rest != nullptr) {
+      d_shs[3 * i] = dsh[0];
+      d_shs[3 * i + 1] = dsh[1];
+      d_shs[3 * i + 2] = dsh[2];
+      float *dr = d_shs_rest + i * ((KS - 1) * 3);
 #pragma unroll
-      for (int j = 0; j < KS * 3 / 4; ++j) {
-        float4 t;
-        t.x = (4 * j + 0 < KE * 3) ? dsh[(4 * j + 0) % (KE * 3)] : 0.0f;
-        t.y = (4 * j + 1 < KE * 3) ? dsh[(4 * j + 1) % (KE * 3)] : 0.0f;
-        t.z = (4 * j + 2 < KE * 3) ? dsh[(4 * j + 2) % (KE * 3)] : 0.0f;
-        t.w = (4 * j + 3 < KE * 3) ? dsh[(4 * j + 3) % (KE * 3)] : 0.0f;
-        d4[j] = t;
-      }
+      for (int j = 3; j < KS * 3; ++j) dr[j - 3] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
     } else {
+      float *ds = d_shs + i * (KS * 3);
+      if constexpr ((KS * 3) % 4 == 0) {
+        float4 *d4 = reinterpret_cast<float4 *>(ds);
 #pragma unroll
-      for (int j = 0; j < KS * 3; ++j) ds[j] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+        for (int j = 0; j < KS * 3 / 4; ++j) {
+          float4 t;
+          t.x = (4 * j + 0 < KE * 3) ? dsh[(4 * j + 0) % (KE * 3)] : 0.0f;
+          t.y = (4 * j + 1 < KE * 3) ? dsh[(4 * j + 1) % (KE * 3)] : 0.0f;
+          t.z = (4 * j + 2 < KE * 3) ? dsh[(4 * j + 2) % (KE * 3)] : 0.0f;
+          t.w = (4 * j + 3 < KE * 3) ? dsh[(4 * j + 3) % (KE * 3)] : 0.0f;
+          d4[j] = t;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < KS * 3; ++j) ds[j] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+      }
     }
   }
   if (d_intr != nullptr || d_extr != nullptr) block_cam_grad(gcam, d_intr, d_extr);
@@ -267,13 +291,13 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
 
 template <int KS>
 int launch_fwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const float *scale,
-                  const float *rot, const float *shs, int64_t n, const float *intr,
+                  const float *rot, const float *shs, const float *shs_rest, int64_t n, const float *intr,
                   const float *extr, const float *center, int W, int H, int gx, int gy,
                   float nearest, float extent, int flags, int C, float *uv, float *depth,
                   float *cov_out, float *conic, int32_t *radius, int32_t *tiles, float *feat) {
 #define GSR_FWD_ARGS                                                                          \
-  xyz, scale, rot, shs, n, intr, extr, center, W, H, gx, gy, nearest, extent, flags, C, uv, \
-      depth, cov_out, conic, radius, tiles, feat
+  xyz, scale, rot, shs, shs_rest, n, intr, extr, center, W, H, gx, gy, nearest, extent, flags, \
+      C, uv, depth, cov_out, conic, radius, tiles, feat
   switch (KE) {
     case 1: preprocess_fwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_fwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break; } return GSR_ERR_ARG;
@@ -287,17 +311,17 @@ int launch_fwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const f
 
 template <int KS>
 int launch_bwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const float *scale,
-                  const float *rot, const float *shs, int64_t n, const float *intr,
+                  const float *rot, const float *shs, const float *shs_rest, int64_t n, const float *intr,
                   const float *extr, const float *center, int W, int H, float nearest,
                   float extent, int flags, int C, const int32_t *radius, const float *d_uv,
                   int ld_uv, const float *d_depth, const float *d_conic, int ld_conic,
                   const float *d_feat, int ld_feat, float *d_xyz,
-                  float *d_scale, float *d_rot, float *d_shs, float *d_intr, float *d_extr,
-                  float *d_center) {
+                  float *d_scale, float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
+                  float *d_extr, float *d_center) {
 #define GSR_BWD_ARGS                                                                          \
-  xyz, scale, rot, shs, n, intr, extr, center, W, H, nearest, extent, flags, C, radius, d_uv, \
-      ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot, d_shs, d_intr, \
-      d_extr, d_center
+  xyz, scale, rot, shs, shs_rest, n, intr, extr, center, W, H, nearest, extent, flags, C,      \
+      radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot,   \
+      d_shs, d_shs_rest, d_intr, d_extr, d_center
   switch (KE) {
     case 1: preprocess_bwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_bwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
@@ -324,7 +348,7 @@ inline int feat_channels(int flags) {
 extern "C" {
 
 int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
-                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, float *uv, float *depth, float *cov3d, float *conic,
                        int32_t *radius, int32_t *tiles, float *feat, void *stream) {
@@ -338,8 +362,8 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
   hipStream_t st = (hipStream_t)stream;
   int rc;
 #define GSR_ARGS                                                                              \
-  KE, nb, st, xyz, scale, rot, shs, n, intr, extr, cam_center, W, H, gx, gy, nearest, extent, \
-      flags, C, uv, depth, cov3d, conic, radius, tiles, feat
+  KE, nb, st, xyz, scale, rot, shs, shs_rest, n, intr, extr, cam_center, W, H, gx, gy, nearest, \
+      extent, flags, C, uv, depth, cov3d, conic, radius, tiles, feat
   switch (K) {
     case 1: rc = launch_fwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_fwd_ks<4>(GSR_ARGS); break;
@@ -352,13 +376,13 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
 }
 
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
-                       int64_t n, int K, int degree, const float *intr, const float *extr,
+                       const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, const int32_t *radius, const float *d_uv, int ld_uv,
                        const float *d_depth, const float *d_conic, int ld_conic,
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
-                       float *d_rot, float *d_shs, float *d_intr, float *d_extr,
-                       float *d_center, void *stream) {
+                       float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
+                       float *d_extr, float *d_center, void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
                   (flags & ~3) == 0,
               "preprocess_bwd: bad arguments (K=%d flags=%d)", K, flags);
@@ -368,9 +392,9 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
   hipStream_t st = (hipStream_t)stream;
   int rc;
 #define GSR_ARGS                                                                               \
-  KE, nb, st, xyz, scale, rot, shs, n, intr, extr, cam_center, W, H, nearest, extent, flags, C, \
-      radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot,   \
-      d_shs, d_intr, d_extr, d_center
+  KE, nb, st, xyz, scale, rot, shs, shs_rest, n, intr, extr, cam_center, W, H, nearest, extent, \
+      flags, C, radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, \
+      d_rot, d_shs, d_shs_rest, d_intr, d_extr, d_center
   switch (K) {
     case 1: rc = launch_bwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_bwd_ks<4>(GSR_ARGS); break;

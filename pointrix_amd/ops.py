@@ -416,16 +416,23 @@ def _aligned16(t: torch.Tensor) -> torch.Tensor:
 
 class _Preprocess(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, xyz, scale, rot, shs, intr, extr, center, W, H, degree, flags, nearest,
-                extent, keep_cov):
-        _dev_check(xyz, scale, rot, shs, intr, extr, center)
+    def forward(ctx, xyz, scale, rot, shs, shs_rest, intr, extr, center, W, H, degree, flags,
+                nearest, extent, keep_cov):
+        _dev_check(xyz, scale, rot, shs, shs_rest, intr, extr, center)
         x, s, q = _f32(xyz), _f32(scale), _f32(rot)
-        sh = _aligned16(_f32(shs))
+        if shs_rest is None:
+            sh, rest = _aligned16(_f32(shs)), None
+            K = sh.shape[1]
+        else:
+            sh, rest = _f32(shs), _f32(shs_rest)
+            if sh.shape[1] != 1:
+                raise ValueError("with shs_rest, shs holds the DC coefficients [N,1,3]")
+            K = 1 + rest.shape[1]
         k, e = _intr4(intr), _extr34(extr)
         c = _f32(center).reshape(-1)[:3].contiguous()
-        n, K = x.shape[0], sh.shape[1]
+        n = x.shape[0]
         if sh.shape[-1] != 3 or K not in (1, 4, 9, 16):
-            raise ValueError(f"shs must be [N,K,3] with K in 1,4,9,16; got {tuple(shs.shape)}")
+            raise ValueError(f"SH must be [N,K,3] with K in 1,4,9,16; got K={K}")
         C = 3 + (1 if flags & FLAG_DEPTH else 0) + (3 if flags & FLAG_NORMAL else 0)
         dev = x.device
         uv = torch.empty(n, 2, device=dev)
@@ -436,21 +443,21 @@ class _Preprocess(torch.autograd.Function):
         feat = torch.empty(n, C, device=dev)
         cov = torch.empty(n, 6, device=dev) if keep_cov else torch.empty(0, device=dev)
         with _span("preprocess_fwd", dev):
-            call("gsr_preprocess_fwd", ptr(x), ptr(s), ptr(q), ptr(sh), n, K, int(degree), ptr(k),
-                 ptr(e), ptr(c), int(W), int(H), float(nearest), float(extent), int(flags),
-                 ptr(uv), ptr(depth), ptr(cov) if keep_cov else None, ptr(conic), ptr(radius),
-                 ptr(tiles), ptr(feat), stream_of(x))
-        ctx.save_for_backward(x, s, q, sh, k, e, c, radius)
+            call("gsr_preprocess_fwd", ptr(x), ptr(s), ptr(q), ptr(sh), ptr(rest), n, K,
+                 int(degree), ptr(k), ptr(e), ptr(c), int(W), int(H), float(nearest),
+                 float(extent), int(flags), ptr(uv), ptr(depth), ptr(cov) if keep_cov else None,
+                 ptr(conic), ptr(radius), ptr(tiles), ptr(feat), stream_of(x))
+        ctx.save_for_backward(x, s, q, sh, rest, k, e, c, radius)
         ctx.cfg = (int(W), int(H), int(degree), int(flags), float(nearest), float(extent), intr,
-                   extr, center, C)
+                   extr, center, C, K)
         ctx.mark_non_differentiable(radius, tiles, cov)
         return uv, depth, conic, radius, tiles, feat, cov
 
     @staticmethod
     def backward(ctx, g_uv, g_depth, g_conic, _gr, _gt, g_feat, _gc):
-        x, s, q, sh, k, e, c, radius = ctx.saved_tensors
-        W, H, degree, flags, nearest, extent, intr, extr, center, C = ctx.cfg
-        n, K = x.shape[0], sh.shape[1]
+        x, s, q, sh, rest, k, e, c, radius = ctx.saved_tensors
+        W, H, degree, flags, nearest, extent, intr, extr, center, C, K = ctx.cfg
+        n = x.shape[0]
         dev = x.device
         z = lambda *shape: torch.zeros(*shape, device=dev)
         g_uv = z(n, 2) if g_uv is None else _rows(g_uv)
@@ -459,18 +466,19 @@ class _Preprocess(torch.autograd.Function):
         g_depth = None if g_depth is None else _f32(g_depth).reshape(-1)
         d_x, d_s, d_q = torch.empty_like(x), torch.empty_like(s), torch.empty_like(q)
         d_sh = torch.empty_like(sh)
-        need_k, need_e, need_c = ctx.needs_input_grad[4], ctx.needs_input_grad[5], \
-            ctx.needs_input_grad[6]
+        d_rest = None if rest is None else torch.empty_like(rest)
+        need_k, need_e, need_c = ctx.needs_input_grad[5], ctx.needs_input_grad[6], \
+            ctx.needs_input_grad[7]
         d_k = z(4) if need_k else None
         d_e = z(12) if need_e else None
         d_c = z(3) if need_c else None
         with _span("preprocess_bwd", dev):
-            call("gsr_preprocess_bwd", ptr(x), ptr(s), ptr(q), ptr(sh), n, K, degree, ptr(k),
-                 ptr(e), ptr(c), W, H, nearest, extent, flags, ptr(radius), ptr(g_uv),
+            call("gsr_preprocess_bwd", ptr(x), ptr(s), ptr(q), ptr(sh), ptr(rest), n, K, degree,
+                 ptr(k), ptr(e), ptr(c), W, H, nearest, extent, flags, ptr(radius), ptr(g_uv),
                  g_uv.stride(0), ptr(g_depth), ptr(g_conic), g_conic.stride(0), ptr(g_feat),
-                 g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_k), ptr(d_e),
-                 ptr(d_c), stream_of(x))
-        return (d_x, d_s, d_q, d_sh, _intr_grad(d_k, intr) if need_k else None,
+                 g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_rest),
+                 ptr(d_k), ptr(d_e), ptr(d_c), stream_of(x))
+        return (d_x, d_s, d_q, d_sh, d_rest, _intr_grad(d_k, intr) if need_k else None,
                 _extr_grad(d_e, extr) if need_e else None,
                 d_c.reshape(center.shape).to(center.dtype) if need_c else None,
                 None, None, None, None, None, None, None)
@@ -482,13 +490,15 @@ def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_mat
     """Everything render_iter does per splat before sorting, in one kernel.
 
     Inputs as MsplatRender.render_iter receives them (activated scaling,
-    unit wxyz rotation, shs [N,K,3]).  Returns (uv [N,2], depth [N,1], conic
+    unit wxyz rotation, shs [N,K,3] -- or the pair (DC [N,1,3], rest
+    [N,K-1,3]), which is read without concatenating).  Returns (uv [N,2], depth [N,1], conic
     [N,3], radius [N], tiles [N], feat [N,C], cov3d [N,6] or empty) where
     feat = clamp(SH colour + 0.5, 0) | depth | camera-space normal.  uv,
     depth, conic, radius and tiles equal those of project_point +
     compute_cov3d + ewa_project.
     """
     flags = (FLAG_DEPTH if depth_channel else 0) | (FLAG_NORMAL if normal_channel else 0)
-    return _Preprocess.apply(position, scaling, rotation, shs, intrinsic_params,
+    shs_dc, shs_rest = (shs[0], shs[1]) if isinstance(shs, (tuple, list)) else (shs, None)
+    return _Preprocess.apply(position, scaling, rotation, shs_dc, shs_rest, intrinsic_params,
                              extrinsic_matrix, camera_center, W, H,
                              -1 if degree is None else degree, flags, nearest, extent, keep_cov)

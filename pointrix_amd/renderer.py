@@ -131,6 +131,11 @@ class MsplatRender:
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
                     position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        """One view (msplat.py:51-158).  ``shs`` is [N,K,3], or the pair
+        (features [N,1,3], features_rest [N,K-1,3]) of pointrix's point cloud,
+        which the fused path reads without concatenating."""
+        if isinstance(shs, (tuple, list)) and not self.cfg.fused:
+            shs = torch.cat(list(shs), dim=1)
         if self.cfg.fused:
             return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
                                            camera_center, position, opacity, scaling, rotation,
@@ -251,6 +256,11 @@ class MsplatNormalRender(MsplatRender):
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
                     position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+        """One view (msplat.py:51-158).  ``shs`` is [N,K,3], or the pair
+        (features [N,1,3], features_rest [N,K-1,3]) of pointrix's point cloud,
+        which the fused path reads without concatenating."""
+        if isinstance(shs, (tuple, list)) and not self.cfg.fused:
+            shs = torch.cat(list(shs), dim=1)
         if self.cfg.fused:
             return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
                                            camera_center, position, opacity, scaling, rotation,

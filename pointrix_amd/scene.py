@@ -103,10 +103,17 @@ class GaussianScene:
         ps = [p.detach().to(device).requires_grad_(requires_grad) for p in self.params()]
         return GaussianScene(*ps)
 
-    def render_inputs(self) -> dict:
-        """Activated tensors, as BaseModel.forward passes them (base_model.py:66-71)."""
-        shs = self.features if self.features_rest is None or self.features_rest.shape[1] == 0 \
-            else torch.cat([self.features, self.features_rest], dim=1)
+    def render_inputs(self, split_sh: bool = False) -> dict:
+        """Activated tensors, as BaseModel.forward passes them (base_model.py:66-71).
+
+        split_sh: pass SH as the pair (features, features_rest) instead of
+        their concatenation (get_shs, gaussian_points.py:86-90)."""
+        if self.features_rest is None or self.features_rest.shape[1] == 0:
+            shs = self.features
+        elif split_sh:
+            shs = (self.features, self.features_rest)
+        else:
+            shs = torch.cat([self.features, self.features_rest], dim=1)
         return {"position": self.position,
                 "opacity": torch.sigmoid(self.opacity),
                 "scaling": torch.exp(self.scaling),

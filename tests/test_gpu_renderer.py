@@ -151,3 +151,27 @@ def test_render_iter_vs_fp64(device, case, snapped, fused):
         assert e < grad_tol, k
     ndc = res["uv_points"].grad
     assert ndc is not None and tuple(ndc.shape) == (dl["position"].shape[0], 2)
+
+
+def test_split_sh_equals_concatenated(device):
+    """shs passed as (features, features_rest) gives the same images and the
+    same gradients as their concatenation."""
+    scene = S.random_scene(3000, 41, sh_degree=3, means="normal", means_std=(0.6, 0.6, 0.6),
+                           log_scale=(-3.0, 0.5), opacity_std=1.5)
+    cam = S.make_camera(96, 80, eye=(0.4, 0.3, -2.5), fov_deg=55.0)
+    outs = []
+    for split in (False, True):
+        sc = scene.to(device, requires_grad=True)
+        r = MsplatRender({"render_depth": True, "render_normal": True}, device="cuda")
+        r.sh_degree = 3
+        res = r.render_iter(**cam.to(device).render_kwargs(), **sc.render_inputs(split_sh=split))
+        imgs = res["rendered_features_split"]
+        sum(v.sum() * (i + 1) for i, v in enumerate(imgs.values())).backward()
+        outs.append(({k: v.detach() for k, v in imgs.items()},
+                     {k: p.grad.detach() for k, p in sc.named_params().items()}))
+    (ia, ga), (ib, gb) = outs
+    for k in ia:
+        assert torch.equal(ia[k], ib[k]), k
+    for k in ga:
+        e = (ga[k] - gb[k]).abs().max().item() / max(ga[k].abs().max().item(), 1e-30)
+        assert e < 1e-5, (k, e)
