@@ -127,6 +127,57 @@ int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, i
 int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
                   int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
                   int32_t *tile_range, void *stream);
+/* m in gsr_sort_emit is the instance capacity of ids_out / ws_inst; the
+ * kernels use min(device count, m), so a fixed capacity (no host read, e.g.
+ * inside a captured graph) is safe: if the device count exceeds it the
+ * emission is truncated in bounds and the caller detects it from *m_out. */
+
+/* ---- multi-view batches (MsplatRender.render_batch, msplat.py:160-213) ---
+ * V cameras of any resolution over one scene, one launch per stage.  View
+ * tables: view_f float[V][20] = intr(4) | extr 3x4 row-major (12) | centre(3)
+ * | pad; view_i int32[V][8] = W, H, gx, gy, first global tile, first pixel,
+ * 0, 0 (tiles of tile_w x tile_h; views' tiles and pixels numbered
+ * consecutively).  Per-view outputs are indexed by j = view * N + splat;
+ * images are packed [C, total_pixels].  V <= 64. */
+int gsr_mv_preprocess_fwd(const float *xyz, const float *scale, const float *rot,
+                          const float *shs, const float *shs_rest, int64_t n, int K, int degree,
+                          int V, const float *view_f, const int32_t *view_i, int tile_w,
+                          int tile_h, float nearest, float extent, int flags, float *uv,
+                          float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                          float *feat, void *stream);
+/* d_splat[V*N, ld]: the records of gsr_mv_alpha_blending_bwd.  Gradients
+ * (position, scale, rotation, opacity, SH) are summed over views and written
+ * once; d_view_f[V][20] (may be NULL) accumulates camera gradients. */
+int gsr_mv_preprocess_bwd(const float *xyz, const float *scale, const float *rot,
+                          const float *shs, const float *shs_rest, int64_t n, int K, int degree,
+                          int V, const float *view_f, const int32_t *view_i, float nearest,
+                          float extent, int flags, const int32_t *radius, const float *d_splat,
+                          int ld, float *d_xyz, float *d_scale, float *d_rot, float *d_opacity,
+                          float *d_shs, float *d_shs_rest, float *d_view_f, void *stream);
+int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
+                      int64_t n_per_view, int V, const int32_t *view_i, int tile_w, int tile_h,
+                      void *ws_gauss, int32_t *m_out, void *stream);
+int gsr_mv_sort_emit(const float *uv, const int32_t *radius, int64_t n_per_view, int V,
+                     const int32_t *view_i, int total_tiles, int tile_w, int tile_h,
+                     void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                     int32_t *tile_range, void *stream);
+/* Persistent blending: min(total_tiles, 4 x CUs) workgroups pull global
+ * tiles from *queue (one device int32 of caller scratch, reset in-stream).
+ * C in {3,4,6,7}; tiles 16x16, 8x32, 32x8; batch_depth 64, 128 or 256 splats
+ * staged per LDS batch. */
+int gsr_mv_alpha_blending_fwd(const float *uv, const float *conic, const float *opacity,
+                              const float *feature, int C, const int32_t *ids,
+                              const int32_t *tile_range, float bg, int64_t n_per_view, int V,
+                              const int32_t *view_i, int total_tiles, int64_t total_pixels,
+                              int tile_w, int tile_h, int batch_depth, int32_t *queue,
+                              float *out, float *T_final, int32_t *n_contrib, void *stream);
+int gsr_mv_alpha_blending_bwd(const float *uv, const float *conic, const float *opacity,
+                              const float *feature, int C, const int32_t *ids,
+                              const int32_t *tile_range, float bg, int64_t n_per_view, int V,
+                              const int32_t *view_i, int total_tiles, int64_t total_pixels,
+                              int tile_w, int tile_h, int batch_depth, int32_t *queue,
+                              const float *T_final, const int32_t *n_contrib, const float *d_out,
+                              const float *d_T_final, float *d_splat, int ld, void *stream);
 
 /* ---- msplat.alpha_blending (msplat.py:148-151) ---------------------------
  * out[C,H,W] = sum_i f_i alpha_i T_i + bg * T_final, front to back over the

@@ -240,9 +240,25 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 // --------------------------------------------------------------------------
 // Binning kernels
 // --------------------------------------------------------------------------
+// Tile grid of the items being binned.  One view: (gx, gy) with vi ==
+// nullptr.  Several views: item j belongs to view j / npv, whose grid and
+// first global tile come from the view table vi (int32 [V][8]: W, H, gx, gy,
+// tile_off, pix_off, -, -); emitted tile ids are global (tile_off + local).
 struct Grid {
   int gx, gy, tw, th;
+  const int32_t *vi;
+  uint32_t npv;
 };
+
+struct ItemGrid {
+  int gx, gy, toff;
+};
+
+__device__ __forceinline__ ItemGrid item_grid(const Grid &g, uint32_t j) {
+  if (g.vi == nullptr) return ItemGrid{g.gx, g.gy, 0};
+  const int32_t *row = g.vi + 8 * (j / g.npv);
+  return ItemGrid{row[2], row[3], row[4]};
+}
 
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
                                                          const int32_t *__restrict__ radius,
@@ -253,7 +269,8 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
   const int r = radius[i];
   uint32_t c = 0;
   if (r > 0) {
-    const Rect q = tile_rect(uv[2 * i], uv[2 * i + 1], r, g.gx, g.gy, g.tw, g.th);
+    const ItemGrid ig = item_grid(g, i);
+    const Rect q = tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
     c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
   }
   cnt[i] = c;
@@ -289,18 +306,22 @@ struct LoadSortedCount {
 __global__ __launch_bounds__(kThreads) void emit_kernel(
     const float *__restrict__ uv, const int32_t *__restrict__ radius,
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
-    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t *__restrict__ tkeys,
-    int32_t *__restrict__ tvals) {
+    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t cap_inst,
+    uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
   const uint32_t nv = count_of(nv_ptr, cap);
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
   if (wave_base >= nv) return;  // uniform per wave
   const int lane = lane_id();
   const uint32_t s = wave_base + lane;
-  int id = 0, x0 = 0, y0 = 0, w = 1, cnt = 0;
+  int id = 0, x0 = 0, y0 = 0, w = 1, cnt = 0, gxi = 0, toff = 0;
   uint32_t off = 0;
   if (s < nv) {
     id = sorted_ids[s];
-    const Rect q = tile_rect(uv[2 * id], uv[2 * id + 1], radius[id], g.gx, g.gy, g.tw, g.th);
+    const ItemGrid ig = item_grid(g, (uint32_t)id);
+    gxi = ig.gx;
+    toff = ig.toff;
+    const Rect q = tile_rect(uv[2 * (int64_t)id], uv[2 * (int64_t)id + 1], radius[id], ig.gx,
+                             ig.gy, g.tw, g.th);
     x0 = q.x0;
     y0 = q.y0;
     w = max(q.x1 - q.x0, 1);
@@ -314,12 +335,17 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
     const int jy0 = __builtin_amdgcn_readlane(y0, j);
     const int jw = __builtin_amdgcn_readlane(w, j);
     const int jcnt = __builtin_amdgcn_readlane(cnt, j);
+    const int jgx = __builtin_amdgcn_readlane(gxi, j);
+    const int jtoff = __builtin_amdgcn_readlane(toff, j);
     const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
     const float inv_w = 1.0f / (float)jw;
-    for (int k = lane; k < jcnt; k += 64) {
+    // capacity guard: a caller that sized the instance buffers below the
+    // device count (graph replays) gets a truncated, in-bounds emission
+    const int kmax = joff >= cap_inst ? 0 : (int)min((uint32_t)jcnt, cap_inst - joff);
+    for (int k = lane; k < kmax; k += 64) {
       const int row = (int)(((float)k + 0.5f) * inv_w);
       const int col = k - row * jw;
-      tkeys[joff + k] = (uint32_t)((jy0 + row) * g.gx + (jx0 + col));
+      tkeys[joff + k] = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
       tvals[joff + k] = jid;
     }
   }
@@ -445,12 +471,12 @@ size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n >
 
 size_t gsr_sort_ws_inst_bytes(int64_t m) { return sortbufs_bytes((uint64_t)(m > 0 ? m : 1)); }
 
-int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,
-                   int W, int H, int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
-                   void *stream) {
-  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
-              "sort_count: bad sizes");
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                    const Grid &gr, void *ws_gauss, int32_t *m_out, hipStream_t s) {
   GaussWs g;
   carve_gauss(ws_gauss, (uint64_t)n, g);
   const uint32_t cap = (uint32_t)(n > 0 ? n : 1);
@@ -459,7 +485,6 @@ int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, i
     fill_kernel<<<1, kThreads, 0, s>>>(m_out, 1, 0);
     return gsr_launch_status("sort_count");
   }
-  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h};
   count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
   // compact visible splats with their depth bits as keys -> Nv in scalars[0]
   run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
@@ -474,14 +499,9 @@ int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, i
   return gsr_launch_status("sort_count");
 }
 
-int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
-                  int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
-                  int32_t *tile_range, void *stream) {
-  GSR_REQUIRE(n >= 0 && m >= 0 && m < (int64_t)INT32_MAX && W > 0 && H > 0,
-              "sort_emit: bad sizes");
-  hipStream_t s = (hipStream_t)stream;
-  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
-  const int T = gx * gy;
+int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid &gr, int T,
+                   void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                   int32_t *tile_range, hipStream_t s) {
   fill_kernel<<<gsr_blocks(2 * (int64_t)T, kThreads), kThreads, 0, s>>>(tile_range, 2 * (int64_t)T,
                                                                         0);
   if (m == 0 || n == 0) return gsr_launch_status("sort_emit");
@@ -489,21 +509,68 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
   carve_gauss(ws_gauss, (uint64_t)n, g);
   SortBufs ib;
   carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
-  const Grid gr{gx, gy, tile_w, tile_h};
   const int nbits = tile_bits(T);
   const int passes = (nbits + kRadixBits - 1) / kRadixBits;
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;
   emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, g.sb.valA, g.offs,
-                                                           g.scalars, (uint32_t)n, gr, ib.keyA,
-                                                           v_first);
+                                                           g.scalars, (uint32_t)n, gr,
+                                                           (uint32_t)m, ib.keyA, v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
   run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
   ranges_kernel<<<gsr_blocks(m, kThreads), kThreads, 0, s>>>(kc, g.scalars + 1, (uint32_t)m,
                                                              tile_range);
   return gsr_launch_status("sort_emit");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                   int W, int H, int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
+                   void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
+              "sort_count: bad sizes");
+  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
+}
+
+int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
+                  int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                  int32_t *tile_range, void *stream) {
+  GSR_REQUIRE(n >= 0 && m >= 0 && m < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 &&
+                  tile_h > 0,
+              "sort_emit: bad sizes");
+  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+  const Grid gr{gx, gy, tile_w, tile_h, nullptr, 0};
+  return sort_emit_impl(uv, radius, n, gr, gx * gy, ws_gauss, m, ws_inst, ids_out, tile_range,
+                        (hipStream_t)stream);
+}
+
+int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
+                      int64_t n_per_view, int V, const int32_t *view_int, int tile_w, int tile_h,
+                      void *ws_gauss, int32_t *m_out, void *stream) {
+  const int64_t n = n_per_view * (int64_t)V;
+  GSR_REQUIRE(V >= 1 && n_per_view >= 0 && n < (int64_t)INT32_MAX && tile_w > 0 && tile_h > 0,
+              "mv_sort_count: bad sizes");
+  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1)};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
+}
+
+int gsr_mv_sort_emit(const float *uv, const int32_t *radius, int64_t n_per_view, int V,
+                     const int32_t *view_int, int total_tiles, int tile_w, int tile_h,
+                     void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                     int32_t *tile_range, void *stream) {
+  const int64_t n = n_per_view * (int64_t)V;
+  GSR_REQUIRE(V >= 1 && n_per_view >= 0 && n < (int64_t)INT32_MAX && m >= 0 &&
+                  m < (int64_t)INT32_MAX && total_tiles > 0,
+              "mv_sort_emit: bad sizes");
+  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1)};
+  return sort_emit_impl(uv, radius, n, gr, total_tiles, ws_gauss, m, ws_inst, ids_out,
+                        tile_range, (hipStream_t)stream);
 }
 
 }  // extern "C"
