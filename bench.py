@@ -77,7 +77,8 @@ class TrainableCamera:
 
 
 def l1(a, b):
-    return (a - b).abs().mean()
+    from pointrix_amd.ops import l1_loss
+    return l1_loss(a, b)
 
 
 def make_step(renderer, scene, cam, targets):
@@ -146,6 +147,107 @@ def cpu_baseline(workload_fn, div, seed=0):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+def bench_batched(steps, warmup, log):
+    """1M splats (SH 3), 16 cameras of uneven resolution,
+    summed L1 over views, fwd+bwd.  Plain = one view per launch
+    (MsplatRender.render_iter per view, one backward); batched = one launch per
+    stage (autotuned tile shape / batch depth); graph = the batched step
+    captured as one HIP graph, every replay checked against eager."""
+    from pointrix_amd import scene as S
+    from pointrix_amd.autotune import autotune, time_step
+    from pointrix_amd.batched import BatchedRender, GraphedStep
+    from pointrix_amd.renderer import MsplatRender
+
+    dev = torch.device("cuda:0")
+    scene_cpu, cams = S.batched1m(0)
+    scene = scene_cpu.to(dev, requires_grad=True)
+    params = scene.params()
+    cd = [c.to(dev).render_kwargs() for c in cams]
+    g = torch.Generator().manual_seed(321)
+    targets = [torch.rand(3, c.height, c.width, generator=g).to(dev) for c in cams]
+    V = len(cams)
+
+    def inputs():
+        return scene.render_inputs(split_sh=True)
+
+    def loss_b(res):
+        return sum(l1(d["rgb"], t) for d, t in zip(res["views"], targets))
+
+    plain = MsplatRender({"max_sh_degree": 3}, device="cuda")
+    plain.sh_degree = 3
+
+    def plain_step():
+        inp = inputs()
+        loss = sum(l1(plain.render_iter(**c, **inp)["rendered_features_split"]["rgb"], t)
+                   for c, t in zip(cd, targets))
+        loss.backward()
+        for p in params:
+            p.grad = None
+
+    def timed(fn, n):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000 / n
+
+    ms_plain = timed(plain_step, steps)
+    log(f"plain {ms_plain:.2f} ms / {V} views")
+    (tile, bd), tune = autotune(cd, inputs, loss_b, params, len(scene), sh_degree=3, reps=2)
+    log(f"autotune -> tile {tile[0]}x{tile[1]} depth {bd}")
+    br = BatchedRender(sh_degree=3, tile=tile, batch_depth=bd)
+
+    def batched_step():
+        loss = loss_b(br.render(cd, **inputs()))
+        loss.backward()
+        for p in params:
+            p.grad = None
+
+    ms_batched = timed(batched_step, steps)
+    log(f"batched {ms_batched:.2f} ms")
+    # eager reference for the graph check
+    loss_e = loss_b(br.render(cd, **inputs()))
+    loss_e.backward()
+    ref_grads = [p.grad.clone() for p in params]
+    loss_e = float(loss_e.detach())
+    for p in params:
+        p.grad = None
+    gs = GraphedStep(br, cd, scene.named_params(), inputs, loss_b).capture()
+    graph_ok = True
+    for _ in range(warmup):
+        gs.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gs.replay()
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t0) * 1000 / steps
+    for _ in range(3):   # every replay against eager
+        gs.replay()
+        torch.cuda.synchronize()
+        dl = abs(float(gs.loss) - loss_e) / abs(loss_e)
+        dg = max(float((p.grad - r).abs().max() / r.abs().max().clamp(min=1e-30))
+                 for p, r in zip(params, ref_grads))
+        if dl > 1e-5 or dg > 1e-4:
+            graph_ok = False
+            log(f"graph replay differs from eager: loss rel {dl:.3g}, grad rel {dg:.3g}")
+    log(f"graph {ms_graph:.2f} ms (replays match eager: {graph_ok})")
+    best = min(ms_batched, ms_graph if graph_ok else float("inf"))
+    return {"workload": "batched1m: 1M splats SH3, 16 views of {256x256, 640x360, 979x546, "
+                        "1237x822, 1920x1080}, summed L1, fwd+bwd",
+            "views": V, "pixels": sum(c.width * c.height for c in cams),
+            "plain_ms": round(ms_plain, 3), "batched_ms": round(ms_batched, 3),
+            "graph_ms": round(ms_graph, 3), "graph_replays_match_eager": graph_ok,
+            "plain_views_per_s": round(V * 1000 / ms_plain, 1),
+            "batched_views_per_s": round(V * 1000 / best, 1),
+            "speedup_vs_plain": round(ms_plain / best, 3),
+            "autotune": {"chosen": f"{tile[0]}x{tile[1]}/{bd}",
+                         "ms": {f"{t[0]}x{t[1]}/{d}": round(v, 3) for (t, d), v in tune.items()}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +256,8 @@ def main():
     ap.add_argument("--workload", default="full3m")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-div", type=int, default=128)
+    ap.add_argument("--batched", type=int, default=1, help="also run the 16-view batched case")
+    ap.add_argument("--only", default="", help="run only this section (batched) -- profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -163,6 +267,12 @@ def main():
     from pointrix_amd import ops
     from pointrix_amd import scene as S
     from pointrix_amd.renderer import MsplatRender
+
+    if args.only == "batched":
+        res = bench_batched(args.steps, args.warmup,
+                            lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        print(json.dumps({"batched": res}), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     wl = S.WORKLOADS[args.workload]
@@ -216,6 +326,12 @@ def main():
                 "traffic": None}
     print(f"[bench] per-kernel: {json.dumps(per_kernel)}", file=sys.stderr, flush=True)
 
+    batched = None
+    if args.batched:
+        print("[bench] batched multi-view ...", file=sys.stderr, flush=True)
+        batched = bench_batched(max(3, args.steps // 4), 2,
+                                lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+
     cpu = None
     if args.cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -234,6 +350,7 @@ def main():
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -200,6 +200,16 @@ int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opa
                            const float *d_out, const float *d_T_final, float *d_splat, int ld,
                            void *stream);
 
+/* ---- fused L1 loss (pointrix/model/loss.py:27, mean |x - y|) -----------
+ * Graph-safe: two-stage fixed-order reduction into *out (device scalar),
+ * using partial[gsr_l1_loss_partials(n)] caller scratch; no atomics, no
+ * memset.  Backward: dx = sign(x - y) * g_out[0] / n. */
+int gsr_l1_loss_partials(int64_t n);
+int gsr_l1_loss_fwd(const float *x, const float *y, int64_t n, float *partial, float *out,
+                    void *stream);
+int gsr_l1_loss_bwd(const float *x, const float *y, int64_t n, const float *g_out, float *dx,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

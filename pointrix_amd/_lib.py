@@ -51,6 +51,9 @@ SIGNATURES = {
                                        _I, _I, _P, _P, _P, _P, _P]),
     "gsr_mv_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I64, _I, _P, _I, _I64, _I,
                                        _I, _I, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "gsr_l1_loss_partials": (_I, [_I64]),
+    "gsr_l1_loss_fwd": (_I, [_P, _P, _I64, _P, _P, _P]),
+    "gsr_l1_loss_bwd": (_I, [_P, _P, _I64, _P, _P, _P]),
     "gsr_alpha_blending_fwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,
                                     _P]),
     "gsr_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,

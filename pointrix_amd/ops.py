@@ -502,3 +502,36 @@ def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_mat
     return _Preprocess.apply(position, scaling, rotation, shs_dc, shs_rest, intrinsic_params,
                              extrinsic_matrix, camera_center, W, H,
                              -1 if degree is None else degree, flags, nearest, extent, keep_cov)
+
+
+# ---------------------------------------------------------------------------
+# Fused L1 loss (pointrix/model/loss.py:27)
+# ---------------------------------------------------------------------------
+class _L1Loss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, y):
+        _dev_check(x, y)
+        if x.shape != y.shape:
+            raise ValueError(f"l1_loss shapes differ: {tuple(x.shape)} vs {tuple(y.shape)}")
+        xc, yc = _f32(x), _f32(y)
+        n = xc.numel()
+        lib = _lib.load()
+        partial = torch.empty(int(lib.gsr_l1_loss_partials(n)), device=xc.device)
+        out = torch.empty((), device=xc.device)
+        call("gsr_l1_loss_fwd", ptr(xc), ptr(yc), n, ptr(partial), ptr(out), stream_of(xc))
+        ctx.save_for_backward(xc, yc)
+        ctx.y_needs = ctx.needs_input_grad[1]
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        xc, yc = ctx.saved_tensors
+        gc = _f32(g)
+        dx = torch.empty_like(xc)
+        call("gsr_l1_loss_bwd", ptr(xc), ptr(yc), xc.numel(), ptr(gc), ptr(dx), stream_of(xc))
+        return dx, (-dx if ctx.y_needs else None)
+
+
+def l1_loss(x, y):
+    """mean |x - y| as one deterministic, graph-capturable reduction."""
+    return _L1Loss.apply(x, y)

@@ -259,3 +259,17 @@ def test_preprocess_backward(device, small, pre32):
         e = rel_err(a.grad, b.grad)
         print(f"ewa d{name}: rel err {e:.3g}")
         assert e < GRAD_TOL, f"ewa d{name}"
+
+
+@pytest.mark.parametrize("n", [1, 1000, 3 * 1920 * 1080])
+def test_l1_loss(device, n):
+    g = torch.Generator().manual_seed(n)
+    x = torch.rand(n, generator=g).to(device).requires_grad_(True)
+    y = torch.rand(n, generator=g).to(device)
+    loss = ops.l1_loss(x, y)
+    loss.backward()
+    x64 = x.detach().double().cpu().requires_grad_(True)
+    ref = (x64 - y.double().cpu()).abs().mean()
+    ref.backward()
+    assert abs(loss.item() - ref.item()) <= 1e-6 * max(1.0, abs(ref.item()))
+    assert torch.allclose(x.grad.double().cpu(), x64.grad, rtol=1e-6, atol=1e-12)
