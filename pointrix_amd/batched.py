@@ -10,20 +10,20 @@ view from one work queue; the backward mirrors it.  Gradients flow into the
 splat parameters, the cameras (when their tensors require grad) and a
 per-view viewspace ``uv_points`` leaf, as in render_batch.
 
-``capture()`` records forward + loss + backward of a fixed camera set into a
-HIP graph (torch.cuda.CUDAGraph) with an instance capacity instead of the
+``GraphedStep`` records forward + loss + backward of a fixed camera set into
+a HIP graph (torch.cuda.CUDAGraph) with an instance capacity instead of the
 host read of the instance count; ``replay()`` reruns it and re-captures with a
 larger capacity if the device count outgrew it.
 """
 from __future__ import annotations
 
-import math
 from dataclasses import dataclass
 
 import torch
 
 from . import _lib, ops
 from ._lib import call, ptr, stream_of
+from .graph import CapturedStep
 
 TILE_SHAPES = ((16, 16), (8, 32), (32, 8))
 BATCH_DEPTHS = (256, 128, 64)
@@ -263,56 +263,26 @@ class BatchedRender:
         return int(self._m_dev.item()) if self._m_dev is not None else 0
 
 
-class GraphedStep:
-    """A fixed camera set's forward + loss + backward captured as one HIP
-    graph.  ``loss_fn(result) -> scalar``; gradients land in the ``.grad`` of
-    the splat leaves (zeroed inside the graph)."""
+class GraphedStep(CapturedStep):
+    """A fixed camera set's forward + loss + backward through a BatchedRender,
+    captured as one HIP graph (pointrix_amd.graph.CapturedStep).
+    ``loss_fn(result) -> scalar``; gradients land in the ``.grad`` of the
+    splat leaves."""
 
     def __init__(self, renderer: BatchedRender, cameras, leaves: dict, inputs_fn, loss_fn,
                  headroom=1.25):
-        self.r, self.cameras, self.leaves = renderer, cameras, leaves
-        self.inputs_fn, self.loss_fn, self.headroom = inputs_fn, loss_fn, headroom
-        self.graph = None
+        self.r, self.cameras = renderer, cameras
+        self.inputs_fn, self.loss_fn = inputs_fn, loss_fn
         self.layout = renderer.layout(cameras)
         dev = next(iter(leaves.values())).device
         self.vi = self.layout.view_int(dev)
         self.half = torch.tensor([[0.5 * w, 0.5 * h] for w, h in
                                   zip(self.layout.widths, self.layout.heights)], device=dev)
+        super().__init__(self._render_loss, leaves, renderer, headroom=headroom)
 
-    def _step(self):
-        for p in self.leaves.values():
-            if p.grad is not None:
-                p.grad.zero_()
+    def _render_loss(self):
         res = self.r.render(self.cameras, **self.inputs_fn(), layout=self.layout, vi=self.vi,
                             half=self.half)
         loss = self.loss_fn(res)
         loss.backward()
         return loss
-
-    def capture(self):
-        r = self.r
-        r.capacity = None
-        self._step()                                   # eager: exact instance count
-        torch.cuda.synchronize()
-        m = r.instances()
-        r.capacity = max(1024, int(math.ceil(m * self.headroom)))
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            for _ in range(2):                         # warm the allocator on the side stream
-                self._step()
-        torch.cuda.current_stream().wait_stream(s)
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.loss = self._step()
-        return self
-
-    def replay(self):
-        """Run the captured step; returns False (and re-captures) if the
-        instance count outgrew the capacity."""
-        self.graph.replay()
-        if self.r.instances() > self.r.capacity:
-            self.capture()
-            self.graph.replay()
-            return False
-        return True

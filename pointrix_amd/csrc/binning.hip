@@ -10,10 +10,12 @@
 // instances in depth order and finishes with ceil(log2(T)/8) stable passes on
 // the tile bits over M.  Result is identical to the full 64-bit sort.
 //
-// Radix passes: 8-bit digits, 4096 items per 256-thread block.  Ranking inside
-// a block uses wave64 ballots (match mask of lanes with equal digits) and
-// per-wave LDS counters, which keeps the sort stable with two barriers per
-// block.  Element counts live on the device so no pass needs a host sync.
+// Radix passes: digits of at most 8 bits (the pass count's bits split evenly),
+// 4096 items per 256-thread block.  Ranking inside a block uses wave64
+// ballots (match mask of lanes with equal digits) and per-wave LDS counters,
+// which keeps the sort stable; the block is reordered by digit in LDS before
+// it is written, so writes go out as runs.  Element counts live on the
+// device so no pass needs a host sync.
 #include "gsr_internal.h"
 
 using namespace gsr;
@@ -153,9 +155,14 @@ struct StoreU32 {
 // --------------------------------------------------------------------------
 // Radix sort passes (32-bit key, 32-bit value, stable)
 // --------------------------------------------------------------------------
+// A pass sorts by the digit (key >> shift) & mask, mask = 2^bits - 1 with
+// bits <= 8; a sort over nbits key bits uses ceil(nbits / 8) passes of equal
+// width (12 tile bits: two passes of 6), which keeps every pass's buckets as
+// few and as long as the pass count allows.
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys,
                                                               const uint32_t *n_ptr,
                                                               uint32_t cap, int shift,
+                                                              uint32_t mask,
                                                               uint32_t *__restrict__ hist,
                                                               uint32_t nb) {
   __shared__ uint32_t s_h[kRadix];
@@ -167,21 +174,32 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
 #pragma unroll 4
     for (int k = 0; k < kItems; ++k) {
       const uint32_t i = base + k * kThreads + threadIdx.x;
-      if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+      if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
     }
   }
   __syncthreads();
   hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// Stable scatter of one block of kTileItems items.  Each wave ranks its
+// items per digit with ballots (match mask of equal digits, rank = earlier
+// lanes of the mask plus the wave's running count); the block then places
+// every item at its digit-sorted position in LDS and writes the block out in
+// LDS order, so consecutive lanes write consecutive addresses of a digit's
+// output run instead of scattering single words.
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
-    uint32_t cap, int shift, const uint32_t *__restrict__ hist_scanned, uint32_t nb) {
-  __shared__ uint32_t s_cnt[kThreads / 64][kRadix];
+    uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb) {
+  static_assert(kRadix == kThreads, "one digit per thread");
+  __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
+  __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
+  __shared__ uint32_t s_key[kTileItems];
+  __shared__ int32_t s_val[kTileItems];
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
   if (base >= n) return;  // uniform over the block
+  const uint32_t mask = (1u << bits) - 1u;
   const int w = threadIdx.x >> 6, lane = lane_id();
 #pragma unroll
   for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][threadIdx.x] = 0;
@@ -197,13 +215,15 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const bool valid = i < n;
     key[r] = valid ? kin[i] : 0u;
     val[r] = valid ? vin[i] : 0;
-    const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+    const uint32_t d = (key[r] >> shift) & mask;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      m &= bit ? bal : ~bal;
+      if (b < bits) {  // uniform
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+      }
     }
     uint32_t prior = 0;
     if (valid) prior = s_cnt[w][d];
@@ -216,23 +236,39 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   __syncthreads();
   {
     const int d = threadIdx.x;
-    uint32_t run = hist_scanned[d * nb + blockIdx.x];
+    uint32_t run = 0;
 #pragma unroll
     for (int k = 0; k < kThreads / 64; ++k) {
       const uint32_t t = s_cnt[k][d];
       s_cnt[k][d] = run;
       run += t;
     }
+    uint32_t total;
+    const uint32_t lstart = block_exclusive_scan(run, total);
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][d] += lstart;
+    s_delta[d] = (int32_t)(hist_scanned[d * nb + blockIdx.x] - lstart);
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t i = wbase + r * 64 + lane;
     if (i < n) {
-      const uint32_t d = (key[r] >> shift) & (kRadix - 1);
-      const uint32_t pos = s_cnt[w][d] + rank[r];
-      kout[pos] = key[r];
-      vout[pos] = val[r];
+      const uint32_t lp = s_cnt[w][(key[r] >> shift) & mask] + rank[r];
+      s_key[lp] = key[r];
+      s_val[lp] = val[r];
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = min((uint32_t)kTileItems, n - base);
+#pragma unroll 4
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = k * kThreads + threadIdx.x;
+    if (i < cnt) {
+      const uint32_t kk = s_key[i];
+      const uint32_t pos = i + (uint32_t)s_delta[(kk >> shift) & mask];
+      kout[pos] = kk;
+      vout[pos] = s_val[i];
     }
   }
 }
@@ -416,6 +452,8 @@ void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap, uint32_t *
   scan_down_kernel<<<nb, kThreads, 0, s>>>(ld, st, n_ptr, cap, partials);
 }
 
+inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBits; }
+
 // Stable LSD passes over bits [0, nbits) of 32-bit keys.  Data starts in
 // (keyA, valA) -- or in (keyA, v0) when v0 != nullptr -- and the final result
 // is returned through the pointers.
@@ -423,11 +461,16 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const uint32_t hn = kRadix * nb;
-  for (int shift = 0; shift < nbits; shift += kRadixBits) {
-    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, b.hist, nb);
+  const int passes = radix_passes(nbits);
+  const int per = passes ? (nbits + passes - 1) / passes : 0;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * per;
+    const int bits = min(per, nbits - shift);
+    const uint32_t mask = (1u << bits) - 1u;
+    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb);
     run_scan(LoadU32{b.hist}, StoreU32{b.hist}, nullptr, hn, b.partials, nullptr, nullptr, s);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(k_cur, v_cur, k_alt, v_alt, n_ptr, cap, shift,
-                                                 b.hist, nb);
+                                                 bits, b.hist, nb);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
@@ -510,7 +553,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   SortBufs ib;
   carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
   const int nbits = tile_bits(T);
-  const int passes = (nbits + kRadixBits - 1) / kRadixBits;
+  const int passes = radix_passes(nbits);
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;

@@ -19,18 +19,11 @@ __device__ __forceinline__ float splat_power(float4 co, float dx, float dy) {
   return fmaf(co.x * dx, dx, fmaf(co.y * dx, dy, (co.z * dy) * dy));
 }
 
-// alpha of a splat, or a negative value when the splat is skipped (exponent
-// positive, or alpha below 1/255).  Shared by forward and backward so the
-// backward replays exactly the forward's decisions.
-__device__ __forceinline__ float splat_alpha(float4 co, float power, float &G) {
-  G = 0.0f;
-  if (power > 0.0f) return -1.0f;
-  if (power < kPowerCut && co.w <= 1.0f) return -1.0f;
-  G = exp_neg(power);
-  const float alpha = fminf(kAlphaMax, co.w * G);
-  if (alpha < kAlphaMin) return -1.0f;
-  return alpha;
-}
+// A splat is skipped at a pixel when its exponent is positive, when the
+// exponent is below kPowerCut with opacity <= 1 (alpha is then certainly below
+// 1/255), or when alpha = min(0.99, opacity * exp(power)) is below 1/255.  The
+// forward and the backward evaluate exactly these tests, so the backward
+// replays the forward's decisions.
 
 // Where a tile lives: its view's image size and tile grid, the tile's index
 // within that grid, the offset of the view's pixels in the (packed) outputs,
@@ -85,25 +78,29 @@ __device__ __forceinline__ void blend_fwd_tile(
       for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
     }
     __syncthreads();
-    if (!done) {
-      const int cnt = min(BD, end - base);
-      for (int k = 0; k < cnt; ++k) {
-        const SplatLds sp = sm.sp[k];
-        const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
-        float G;
-        const float alpha = splat_alpha(sp.co, splat_power(sp.co, dx, dy), G);
-        if (alpha < 0.0f) continue;
-        const float test_T = T * (1.0f - alpha);
-        if (test_T < kTMin) {
-          done = true;
-          break;
-        }
-        const float w = alpha * T;
+    const int cnt = min(BD, end - base);
+    for (int k = 0; k < cnt; ++k) {
+      // Branch-light walk: lanes keep their own state through selects; the
+      // wave only branches (uniformly) to skip a splat no active lane needs
+      // or to leave once every lane is done.  Decisions are exactly those of
+      // the straightforward loop (a lane that skips or stops changes nothing).
+      if (__ballot(!done) == 0ull) break;
+      const SplatLds sp = sm.sp[k];
+      const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
+      const float power = splat_power(sp.co, dx, dy);
+      const bool near = !done && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
+      if (__ballot(near) == 0ull) continue;
+      const float alpha = fminf(kAlphaMax, sp.co.w * exp_neg(power));
+      const bool valid = near && !(alpha < kAlphaMin);
+      const float test_T = T * (1.0f - alpha);
+      const bool stop = valid && test_T < kTMin;
+      const bool contrib = valid && !stop;
+      done = done || stop;
+      const float w = contrib ? alpha * T : 0.0f;
 #pragma unroll
-        for (int c = 0; c < C; ++c) acc[c] = fmaf(sm.f[c * BD + k], w, acc[c]);
-        T = test_T;
-        last = base - start + k + 1;
-      }
+      for (int c = 0; c < C; ++c) acc[c] = fmaf(sm.f[c * BD + k], w, acc[c]);
+      T = contrib ? test_T : T;
+      last = contrib ? base - start + k + 1 : last;
     }
   }
   if (inside) {
@@ -195,41 +192,47 @@ __device__ __forceinline__ void blend_bwd_tile(
       float v[QP];
 #pragma unroll
       for (int q = 0; q < QP; ++q) v[q] = 0.0f;
-      bool contrib = local < last;
-      if (contrib) {
-        const SplatLds sp = sm.sp[k];
-        const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
-        float G;
-        const float alpha = splat_alpha(sp.co, splat_power(sp.co, dx, dy), G);
-        contrib = alpha >= 0.0f;
-        if (contrib) {
-          const float inv_om = __builtin_amdgcn_rcpf(1.0f - alpha);
-          T = T * inv_om;
-          const float w = alpha * T;
-          float dL_dalpha = 0.0f;
-#pragma unroll
-          for (int c = 0; c < C; ++c) {
-            const float f = sm.f[c * BD + k];
-            v[6 + c] = w * gpix[c];
-            accum[c] = last_alpha * last_f[c] + (1.0f - last_alpha) * accum[c];
-            last_f[c] = f;
-            dL_dalpha += (f - accum[c]) * gpix[c];
-          }
-          dL_dalpha *= T;
-          dL_dalpha -= T_final * inv_om * bg_term;
-          last_alpha = alpha;
-          if (sp.co.w * G < kAlphaMax) {  // alpha not clamped
-            v[5] = G * dL_dalpha;
-            const float dpow = alpha * dL_dalpha;
-            v[0] = dpow * (2.0f * sp.co.x * dx + sp.co.y * dy);
-            v[1] = dpow * (sp.co.y * dx + 2.0f * sp.co.z * dy);
-            v[2] = dpow * (-0.5f * dx * dx);
-            v[3] = dpow * (-dx * dy);
-            v[4] = dpow * (-0.5f * dy * dy);
-          }
-        }
-      }
+      // same decisions as the forward, evaluated branch-light
+      const SplatLds sp = sm.sp[k];
+      const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
+      const float power = splat_power(sp.co, dx, dy);
+      const bool near = local < last && !(power > 0.0f) &&
+                        !(power < kPowerCut && sp.co.w <= 1.0f);
+      if (__ballot(near) == 0ull) continue;  // wave-uniform
+      // lanes that do not use this splat evaluate exp at 0 (power may be
+      // large and positive there, and G feeds a product that must stay finite)
+      const float G = exp_neg(near ? power : 0.0f);
+      const float alpha = fminf(kAlphaMax, sp.co.w * G);
+      const bool contrib = near && !(alpha < kAlphaMin);
       if (__ballot(contrib) == 0ull) continue;  // wave-uniform
+      {
+        const float inv_om = __builtin_amdgcn_rcpf(1.0f - alpha);
+        const float Tn = T * inv_om;
+        T = contrib ? Tn : T;
+        const float w = contrib ? alpha * Tn : 0.0f;
+        float dL_dalpha = 0.0f;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float f = sm.f[c * BD + k];
+          v[6 + c] = w * gpix[c];
+          const float acc_n = last_alpha * last_f[c] + (1.0f - last_alpha) * accum[c];
+          accum[c] = contrib ? acc_n : accum[c];
+          last_f[c] = contrib ? f : last_f[c];
+          dL_dalpha += (f - acc_n) * gpix[c];
+        }
+        dL_dalpha *= Tn;
+        dL_dalpha -= T_final * inv_om * bg_term;
+        last_alpha = contrib ? alpha : last_alpha;
+        // alpha = min(0.99, o G): no gradient where clamped
+        const float gsel = (contrib && sp.co.w * G < kAlphaMax) ? dL_dalpha : 0.0f;
+        v[5] = G * gsel;
+        const float dpow = alpha * gsel;
+        v[0] = dpow * (2.0f * sp.co.x * dx + sp.co.y * dy);
+        v[1] = dpow * (sp.co.y * dx + 2.0f * sp.co.z * dy);
+        v[2] = dpow * (-0.5f * dx * dx);
+        v[3] = dpow * (-dx * dy);
+        v[4] = dpow * (-0.5f * dy * dy);
+      }
       Bfly<QP, 32>::run(v, lane);
       if (writer) atomicAdd(d_splat + (int64_t)sm.id[k] * ld + my_q, v[0]);
     }

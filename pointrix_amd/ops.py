@@ -222,11 +222,19 @@ def ewa_project(xyz, cov3d, intr, extr, uv, W, H, visible):
 # sort_gaussian  (msplat.py:133-135) -- not differentiable
 # ---------------------------------------------------------------------------
 class SortWorkspace:
-    """Reusable device workspace for the binning stage (grows, never shrinks)."""
+    """Reusable device workspace for the binning stage (grows, never shrinks).
+
+    ``capacity`` = None sizes the instance buffers exactly, with one host read
+    of the device instance count per sort.  An integer capacity skips that
+    read (as a captured graph requires): the instance list is allocated at the
+    capacity, emission never writes past it, and ``instances()`` returns the
+    true count of the last sort so the caller can detect an overflow."""
 
     def __init__(self):
         self.gauss = None
         self.inst = None
+        self.m_dev = None
+        self.capacity = None
 
     def gauss_buf(self, n, device):
         need = int(_lib.load().gsr_sort_ws_gauss_bytes(n))
@@ -240,6 +248,15 @@ class SortWorkspace:
             self.inst = torch.empty(need, dtype=torch.uint8, device=device)
         return self.inst
 
+    def count_buf(self, device):
+        if self.m_dev is None or self.m_dev.device != device:
+            self.m_dev = torch.zeros(1, dtype=torch.int32, device=device)
+        return self.m_dev
+
+    def instances(self) -> int:
+        """Device instance count of the last sort (host read)."""
+        return int(self.m_dev.item()) if self.m_dev is not None else 0
+
 
 _default_ws = SortWorkspace()
 last_instance_count = 0  # M of the most recent sort_gaussian call (diagnostics)
@@ -249,7 +266,9 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     """-> (gaussian_ids_sorted [M] int32, tile_range [T,2] int32).
 
     Order: (tile, float32 bits of depth, splat index).  One host read of the
-    instance count M sizes the outputs.  ``tiles`` is accepted for msplat
+    instance count M sizes the outputs, unless the workspace carries a fixed
+    capacity (then ids has that length and only its first M entries, the
+    ones tile_range points at, are meaningful).  ``tiles`` is accepted for msplat
     signature compatibility; counts are recomputed from (uv, radius), which is
     what ewa_project's ``tiles`` holds.
     """
@@ -262,14 +281,17 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     dev = u.device
     st = stream_of(u)
     gx, gy = (W + tile_w - 1) // tile_w, (H + tile_h - 1) // tile_h
-    m_dev = torch.empty(1, dtype=torch.int32, device=dev)
+    m_dev = ws.count_buf(dev)
     g = ws.gauss_buf(n, dev)
     with _span("sort_count", dev):
         call("gsr_sort_count", ptr(u), ptr(d), ptr(r), n, int(W), int(H), tile_w, tile_h,
              ptr(g), ptr(m_dev), st)
-    m = int(m_dev.item())
-    global last_instance_count
-    last_instance_count = m
+    if ws.capacity is None:
+        m = int(m_dev.item())
+        global last_instance_count
+        last_instance_count = m
+    else:
+        m = int(ws.capacity)
     ids = torch.empty(m, dtype=torch.int32, device=dev)
     tile_range = torch.empty(gx * gy, 2, dtype=torch.int32, device=dev)
     inst = ws.inst_buf(m, dev)
@@ -322,7 +344,7 @@ class _AlphaBlending(torch.autograd.Function):
         d_u, d_c, d_o, d_f = rec[:, 0:2], rec[:, 2:5], rec[:, 5], rec[:, 6:6 + C]
         d_ndc = None
         if ctx.needs_input_grad[9]:
-            d_ndc = d_u * torch.tensor([0.5 * W, 0.5 * H], device=dev)
+            d_ndc = torch.stack((d_u[:, 0] * (0.5 * W), d_u[:, 1] * (0.5 * H)), 1)
         return (d_u, d_c, d_o.reshape(op_shape), d_f, None, None, None, None, None, d_ndc, None,
                 None)
 
