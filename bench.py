@@ -212,7 +212,7 @@ def cpu_baseline(workload_fn, div, seed=0):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
-def bench_batched(steps, warmup, log):
+def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
     """1M splats (SH 3), 16 cameras of uneven resolution,
     summed L1 over views, fwd+bwd.  Plain = one view per launch
     (MsplatRender.render_iter per view, one backward); batched = one launch per
@@ -224,7 +224,7 @@ def bench_batched(steps, warmup, log):
     from pointrix_amd.renderer import MsplatRender
 
     dev = torch.device("cuda:0")
-    scene_cpu, cams = S.batched1m(0)
+    scene_cpu, cams = S.batched1m(0, n=n)
     scene = scene_cpu.to(dev, requires_grad=True)
     params = scene.params()
     cd = [c.to(dev).render_kwargs() for c in cams]
@@ -261,8 +261,11 @@ def bench_batched(steps, warmup, log):
 
     ms_plain = timed(plain_step, steps)
     log(f"plain {ms_plain:.2f} ms / {V} views")
-    (tile, bd), tune = autotune(cd, inputs, loss_b, params, len(scene), sh_degree=3, reps=2)
-    log(f"autotune -> tile {tile[0]}x{tile[1]} depth {bd}")
+    if tune is None:
+        (tile, bd), tune = autotune(cd, inputs, loss_b, params, len(scene), sh_degree=3, reps=2)
+        log(f"autotune -> tile {tile[0]}x{tile[1]} depth {bd}")
+    else:
+        (tile, bd), tune = tune
     br = BatchedRender(sh_degree=3, tile=tile, batch_depth=bd)
 
     def batched_step():
@@ -314,8 +317,8 @@ def bench_batched(steps, warmup, log):
                 f"(eager spread {noise:.3g})")
     log(f"graph {ms_graph:.2f} ms (replays match eager: {graph_ok})")
     best = min(ms_batched, ms_graph if graph_ok else float("inf"))
-    return {"workload": "batched1m: 1M splats SH3, 16 views of {256x256, 640x360, 979x546, "
-                        "1237x822, 1920x1080}, summed L1, fwd+bwd",
+    return {"workload": f"batched: {n // 1_000_000}M splats SH3, 16 views of {{256x256, "
+                        "640x360, 979x546, 1237x822, 1920x1080}, summed L1, fwd+bwd",
             "views": V, "pixels": sum(c.width * c.height for c in cams),
             "plain_ms": round(ms_plain, 3), "batched_ms": round(ms_batched, 3),
             "graph_ms": round(ms_graph, 3), "graph_replays_match_eager": graph_ok,
@@ -324,7 +327,46 @@ def bench_batched(steps, warmup, log):
             "batched_views_per_s": round(V * 1000 / best, 1),
             "speedup_vs_plain": round(ms_plain / best, 3),
             "autotune": {"chosen": f"{tile[0]}x{tile[1]}/{bd}",
-                         "ms": {f"{t[0]}x{t[1]}/{d}": round(v, 3) for (t, d), v in tune.items()}}}
+                         "ms": {f"{t[0]}x{t[1]}/{d}": round(v, 3) for (t, d), v in tune.items()}},
+            "_tune": ((tile, bd), tune)}
+
+
+def bench_serve(log, n_requests=300, rates=(50.0, 200.0, 800.0)):
+    """Serving: 2M resident splats (SH 3), Poisson request arrivals at each
+    rate, resolutions {540p 40%, 720p 30%, 1080p 25%, 2160p 5%}, eye at
+    distance U[2,8], 20% of requests also want depth.  The same trace is
+    replayed in real time against the one-view-per-call server (plain) and
+    the continuous-batching server; latency is arrival to images complete."""
+    from pointrix_amd import scene as S
+    from pointrix_amd.serve import BatchServer, PlainServer, make_requests, run
+
+    dev = torch.device("cuda:0")
+    scene_cpu, _ = S.serve2m(0)
+    scene = scene_cpu.to(dev)
+    with torch.no_grad():
+        inputs = scene.render_inputs(split_sh=True)
+    servers = {"plain": PlainServer(inputs), "batched": BatchServer(inputs)}
+    warm = make_requests(1000.0, 24, seed=99)
+    for srv in servers.values():               # allocator / code warm-up at every size
+        run(srv, warm)
+    out = {}
+    for rate in rates:
+        reqs = make_requests(rate, n_requests, seed=int(rate))
+        row = {}
+        for name, srv in servers.items():
+            st = run(srv, reqs)
+            row[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+            log(f"serve {rate:.0f} req/s {name}: p50 {st['p50_ms']:.2f} ms p99 "
+                f"{st['p99_ms']:.2f} ms, {st['views_per_s']:.1f} views/s, mean batch "
+                f"{st['mean_batch']:.2f}")
+        row["p99_speedup"] = round(row["plain"]["p99_ms"] / row["batched"]["p99_ms"], 3)
+        out[f"{rate:.0f}"] = row
+    b = servers["batched"]
+    return {"workload": "serve2m: 2M splats SH3, Poisson arrivals, {540p .40, 720p .30, "
+                        "1080p .25, 2160p .05}, eye r U[2,8], 20% depth",
+            "requests_per_rate": n_requests,
+            "key_budget_instances": b.max_instances, "budget_splits": b.splits,
+            "rates": out}
 
 
 def main():
@@ -336,11 +378,13 @@ def main():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-div", type=int, default=128)
     ap.add_argument("--batched", type=int, default=1, help="also run the 16-view batched case")
+    ap.add_argument("--serve", type=int, default=1, help="also run the serving trace")
     ap.add_argument("--graph", type=int, default=1,
                     help="also time the step captured as one HIP graph (value = the faster "
                          "of eager and graph, the graph only if its replays match eager)")
     ap.add_argument("--only", default="",
-                    help="run only this section: batched, or copy (HBM counter calibration)")
+                    help="run only this section: batched, serve, or copy (HBM counter "
+                         "calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -361,9 +405,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 4}), flush=True)
         return
+    if args.only == "serve":
+        res = bench_serve(lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        print(json.dumps({"serving": res}), flush=True)
+        return
     if args.only == "batched":
         res = bench_batched(args.steps, args.warmup,
                             lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        res.pop("_tune")
         print(json.dumps({"batched": res}), flush=True)
         return
 
@@ -419,6 +468,22 @@ def main():
                 "traffic": None}
     print(f"[bench] per-kernel: {json.dumps(per_kernel)}", file=sys.stderr, flush=True)
 
+    # the same step with opacity-aware binning (same images, fewer instances;
+    # not the msplat tile contract, so reported beside the headline only)
+    renderer.cfg.tight_tiles = True
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_tight = (time.perf_counter() - t0) * 1000 / args.steps
+    m_tight = ops.last_instance_count
+    renderer.cfg.tight_tiles = False
+    print(f"[bench] tight tiles: {ms_tight:.3f} ms/step, {m_tight} instances",
+          file=sys.stderr, flush=True)
+
     graph = None
     if args.graph:
         print("[bench] graphed step ...", file=sys.stderr, flush=True)
@@ -431,8 +496,18 @@ def main():
     batched = None
     if args.batched:
         print("[bench] batched multi-view ...", file=sys.stderr, flush=True)
-        batched = bench_batched(max(3, args.steps // 4), 2,
-                                lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        blog = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+        b1 = bench_batched(max(3, args.steps // 4), 2, blog)
+        print("[bench] batched multi-view at 3M splats ...", file=sys.stderr, flush=True)
+        b3 = bench_batched(3, 1, blog, n=3_000_000, tune=b1.pop("_tune"))
+        b3.pop("_tune")
+        b3["autotune"] = {"chosen": b1["autotune"]["chosen"], "note": "choice of the 1M run"}
+        batched = {"1M": b1, "3M": b3}
+
+    serving = None
+    if args.serve:
+        print("[bench] serving ...", file=sys.stderr, flush=True)
+        serving = bench_serve(lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
 
     cpu = None
     if args.cpu_baseline:
@@ -450,10 +525,12 @@ def main():
                    "splats": N, "width": W, "height": H, "visible": Nv, "instances": M,
                    "mpixel_per_s": round(P / 1e6 * 1000.0 / ms, 2),
                    "eager_ms_per_step": round(ms_eager, 4), "graph": graph,
+                   "tight_tiles": {"ms_per_step": round(ms_tight, 4), "instances": m_tight},
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -132,6 +132,23 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
  * inside a captured graph) is safe: if the device count exceeds it the
  * emission is truncated in bounds and the caller detects it from *m_out. */
 
+/* Opacity-aware ("tight") binning, no counterpart in msplat: as
+ * gsr_sort_count / gsr_sort_emit, but a splat is emitted only into the tiles
+ * of its rectangle where it can reach alpha >= 1/255 at some pixel centre
+ * (a conservative test on the minimum of its conic's quadratic form over the
+ * tile; conic [N,3] and opacity [N] as the blend reads them).  Blending the
+ * result gives the same images bit for bit with fewer instances; ids_out and
+ * tile_range then differ from the msplat contract (each tile's list is a
+ * subsequence of the exact one).  conic == opacity == NULL is the exact
+ * contract. */
+int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *radius,
+                         const float *conic, const float *opacity, int64_t n, int W, int H,
+                         int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, void *stream);
+int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *conic,
+                        const float *opacity, int64_t n, int W, int H, int tile_w, int tile_h,
+                        void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                        int32_t *tile_range, void *stream);
+
 /* ---- multi-view batches (MsplatRender.render_batch, msplat.py:160-213) ---
  * V cameras of any resolution over one scene, one launch per stage.  View
  * tables: view_f float[V][20] = intr(4) | extr 3x4 row-major (12) | centre(3)
@@ -154,13 +171,16 @@ int gsr_mv_preprocess_bwd(const float *xyz, const float *scale, const float *rot
                           float extent, int flags, const int32_t *radius, const float *d_splat,
                           int ld, float *d_xyz, float *d_scale, float *d_rot, float *d_opacity,
                           float *d_shs, float *d_shs_rest, float *d_view_f, void *stream);
+/* conic [V*N,3] / opacity [N]: both NULL for the exact tile contract, both
+ * set for tight binning (see gsr_sort_count_tight). */
 int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
-                      int64_t n_per_view, int V, const int32_t *view_i, int tile_w, int tile_h,
-                      void *ws_gauss, int32_t *m_out, void *stream);
-int gsr_mv_sort_emit(const float *uv, const int32_t *radius, int64_t n_per_view, int V,
-                     const int32_t *view_i, int total_tiles, int tile_w, int tile_h,
-                     void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
-                     int32_t *tile_range, void *stream);
+                      const float *conic, const float *opacity, int64_t n_per_view, int V,
+                      const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,
+                      int32_t *m_out, void *stream);
+int gsr_mv_sort_emit(const float *uv, const int32_t *radius, const float *conic,
+                     const float *opacity, int64_t n_per_view, int V, const int32_t *view_i,
+                     int total_tiles, int tile_w, int tile_h, void *ws_gauss, int64_t m,
+                     void *ws_inst, int32_t *ids_out, int32_t *tile_range, void *stream);
 /* Persistent blending: min(total_tiles, 4 x CUs) workgroups pull global
  * tiles from *queue (one device int32 of caller scratch, reset in-stream).
  * C in {3,4,6,7}; tiles 16x16, 8x32, 32x8; batch_depth 64, 128 or 256 splats

@@ -283,6 +283,97 @@ int64_t ref_sort(const float *uv, const float *depth, const int32_t *radius, int
   return m;
 }
 
+/* ---- opacity-aware ("tight") binning -----------------------------------
+ * Extension of the build with no msplat counterpart.  A splat is kept in a
+ * tile of its rectangle when o * exp(-q/2) >= 1/255 can hold at some pixel
+ * centre of the tile, q = a dx^2 + 2 b dx dy + c dy^2 with d = uv - pixel:
+ * q's minimum over the tile's pixel-centre box (0 with the centre inside,
+ * else the least of the four edge minima, each edge a 1-D quadratic clamped
+ * to the edge), lowered by the margin 1e-3 + 1e-4 * S (S bounds the sum of
+ * the terms' magnitudes over the box), decides.  Same operations, in the same
+ * order, as the kernels; the blend is checked to give identical images. */
+static float quad_edge_min(float a, float b, float c, float X, float y0, float y1) {
+  const float y = fminf(fmaxf(-(b * X) / c, y0), y1);
+  return a * X * X + 2.0f * b * X * y + c * y * y;
+}
+
+static int tile_reachable(float u, float v, float a, float b, float c, float o, float x0,
+                          float x1, float y0, float y1) {
+  const float alpha_min = 1.0f / 255.0f;
+  float dx0, dx1, dy0, dy1, m, DX, DY, S, t;
+  if (!(a > 0.0f) || !(c > 0.0f)) return 1;
+  dx0 = u - x1;
+  dx1 = u - x0;
+  dy0 = v - y1;
+  dy1 = v - y0;
+  if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return 1;
+  m = quad_edge_min(a, b, c, dx0, dy0, dy1);
+  m = fminf(m, quad_edge_min(a, b, c, dx1, dy0, dy1));
+  m = fminf(m, quad_edge_min(c, b, a, dy0, dx0, dx1));
+  m = fminf(m, quad_edge_min(c, b, a, dy1, dx0, dx1));
+  DX = fmaxf(fabsf(dx0), fabsf(dx1));
+  DY = fmaxf(fabsf(dy0), fabsf(dy1));
+  S = fabsf(a) * DX * DX + 2.0f * fabsf(b) * DX * DY + fabsf(c) * DY * DY;
+  t = fmaxf(m - (1e-3f + 1e-4f * S), 0.0f);
+  return o * exp_neg(-0.5f * t) >= alpha_min;
+}
+
+int ref_tile_reachable(float u, float v, float a, float b, float c, float o, float x0, float x1,
+                       float y0, float y1) {
+  return tile_reachable(u, v, a, b, c, o, x0, x1, y0, y1);
+}
+
+static int tile_keep(const float *uv, const float *conic, const float *opacity, int64_t i,
+                     int tx, int ty, int tw, int th, int W, int H) {
+  const int px0 = tx * tw, py0 = ty * th;
+  const int px1 = (px0 + tw < W ? px0 + tw : W) - 1, py1 = (py0 + th < H ? py0 + th : H) - 1;
+  return tile_reachable(uv[2 * i], uv[2 * i + 1], conic[3 * i], conic[3 * i + 1],
+                        conic[3 * i + 2], opacity[i], (float)px0, (float)px1, (float)py0,
+                        (float)py1);
+}
+
+/* as ref_sort, keeping only the tiles tile_keep accepts; returns M */
+int64_t ref_sort_tight(const float *uv, const float *depth, const int32_t *radius,
+                       const float *conic, const float *opacity, int64_t n, int W, int H, int tw,
+                       int th, int32_t *ids, int32_t *ranges) {
+  const int gx = (W + tw - 1) / tw, gy = (H + th - 1) / th;
+  int64_t m = 0, o = 0, j;
+  Inst *v;
+  for (int64_t i = 0; i < n; ++i) {
+    int x0, x1, y0, y1, tx, ty;
+    if (radius[i] <= 0) continue;
+    rect_of(uv[2 * i], uv[2 * i + 1], radius[i], gx, gy, tw, th, &x0, &x1, &y0, &y1);
+    for (ty = y0; ty < y1; ++ty)
+      for (tx = x0; tx < x1; ++tx) m += tile_keep(uv, conic, opacity, i, tx, ty, tw, th, W, H);
+  }
+  v = (Inst *)malloc(sizeof(Inst) * (size_t)(m > 0 ? m : 1));
+  for (int64_t i = 0; i < n; ++i) {
+    int x0, x1, y0, y1, tx, ty;
+    uint32_t bits;
+    if (radius[i] <= 0) continue;
+    memcpy(&bits, depth + i, 4);
+    rect_of(uv[2 * i], uv[2 * i + 1], radius[i], gx, gy, tw, th, &x0, &x1, &y0, &y1);
+    for (ty = y0; ty < y1; ++ty)
+      for (tx = x0; tx < x1; ++tx) {
+        if (!tile_keep(uv, conic, opacity, i, tx, ty, tw, th, W, H)) continue;
+        v[o].key = ((uint64_t)(uint32_t)(ty * gx + tx) << 32) | bits;
+        v[o].seq = (uint32_t)o;
+        v[o].id = (int32_t)i;
+        ++o;
+      }
+  }
+  qsort(v, (size_t)m, sizeof(Inst), inst_cmp);
+  memset(ranges, 0, sizeof(int32_t) * 2 * (size_t)gx * gy);
+  for (j = 0; j < m; ++j) {
+    const uint32_t t = (uint32_t)(v[j].key >> 32);
+    ids[j] = v[j].id;
+    if (j == 0 || (uint32_t)(v[j - 1].key >> 32) != t) ranges[2 * t] = (int32_t)j;
+    if (j == m - 1 || (uint32_t)(v[j + 1].key >> 32) != t) ranges[2 * t + 1] = (int32_t)(j + 1);
+  }
+  free(v);
+  return m;
+}
+
 /* ---- blending ----------------------------------------------------------- */
 void ref_blend(const float *uv, const float *conic, const float *opacity, const float *feat,
                int C, const int32_t *ids, const int32_t *ranges, float bg, int W, int H, int tw,

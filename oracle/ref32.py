@@ -32,6 +32,10 @@ def lib():
         L.ref_sort_count.argtypes = [P, P, I64, I, I, I, I]
         L.ref_sort.restype = I64
         L.ref_sort.argtypes = [P, P, P, I64, I, I, I, I, P, P]
+        L.ref_tile_reachable.restype = I
+        L.ref_tile_reachable.argtypes = [F] * 10
+        L.ref_sort_tight.restype = I64
+        L.ref_sort_tight.argtypes = [P, P, P, P, P, I64, I, I, I, I, P, P]
         L.ref_blend.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P]
         L.ref_preprocess.argtypes = [P, P, P, P, I64, I, I, P, P, P, I, I, F, F, I, P, P, P, P, P,
                                      P]
@@ -105,6 +109,28 @@ def sort_gaussian(uv, depth, W, H, radius, tile_w=16, tile_h=16):
     rng = np.zeros((gx * gy, 2), np.int32)
     lib().ref_sort(_p(u), _p(d), _p(r), n, W, H, tile_w, tile_h, _p(ids), _p(rng))
     return ids[:m], rng
+
+
+def tile_reachable(u, v, a, b, c, o, x0, x1, y0, y1) -> bool:
+    """The tight-binning test of one splat against one pixel-centre box."""
+    return bool(lib().ref_tile_reachable(*[float(t) for t in (u, v, a, b, c, o, x0, x1, y0,
+                                                                y1)]))
+
+
+def sort_gaussian_tight(uv, depth, W, H, radius, conic, opacity, tile_w=16, tile_h=16):
+    """Opacity-aware binning -> (ids [M] int32, tile_range [T,2] int32)."""
+    u, d, r = _c(uv, np.float32), _c(np.asarray(depth).reshape(-1), np.float32), \
+        _c(np.asarray(radius).reshape(-1), np.int32)
+    cn, op = _c(conic, np.float32), _c(np.asarray(opacity).reshape(-1), np.float32)
+    n = u.shape[0]
+    gx, gy = (W + tile_w - 1) // tile_w, (H + tile_h - 1) // tile_h
+    # the count pass is repeated inside; size ids by the exact (rect) count
+    cap = int(lib().ref_sort_count(_p(u), _p(r), n, W, H, tile_w, tile_h))
+    ids = np.zeros(max(cap, 1), np.int32)
+    rng = np.zeros((gx * gy, 2), np.int32)
+    m = lib().ref_sort_tight(_p(u), _p(d), _p(r), _p(cn), _p(op), n, W, H, tile_w, tile_h,
+                             _p(ids), _p(rng))
+    return ids[:m].copy(), rng
 
 
 def alpha_blending(uv, conic, opacity, feature, ids, tile_range, bg, W, H, tile_w=16, tile_h=16):

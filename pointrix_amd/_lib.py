@@ -45,8 +45,12 @@ SIGNATURES = {
                                    _I, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_mv_preprocess_bwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _F, _F, _I, _P,
                                    _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
-    "gsr_mv_sort_count": (_I, [_P, _P, _P, _I64, _I, _P, _I, _I, _P, _P, _P]),
-    "gsr_mv_sort_emit": (_I, [_P, _P, _I64, _I, _P, _I, _I, _I, _P, _I64, _P, _P, _P, _P]),
+    "gsr_sort_count_tight": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),
+    "gsr_sort_emit_tight": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _I64, _P, _P, _P,
+                                 _P]),
+    "gsr_mv_sort_count": (_I, [_P, _P, _P, _P, _P, _I64, _I, _P, _I, _I, _P, _P, _P]),
+    "gsr_mv_sort_emit": (_I, [_P, _P, _P, _P, _I64, _I, _P, _I, _I, _I, _P, _I64, _P, _P, _P,
+                              _P]),
     "gsr_mv_alpha_blending_fwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I64, _I, _P, _I, _I64, _I,
                                        _I, _I, _P, _P, _P, _P, _P]),
     "gsr_mv_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I64, _I, _P, _I, _I64, _I,

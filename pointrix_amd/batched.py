@@ -72,6 +72,15 @@ def view_floats(cameras) -> torch.Tensor:
     return torch.stack(rows)
 
 
+class InstanceBudgetExceeded(RuntimeError):
+    """The batch needs more (splat, tile) instances than the renderer's
+    ``max_instances`` allows; raised after counting, before allocating."""
+
+    def __init__(self, needed: int, budget: int):
+        super().__init__(f"batch needs {needed} instances, budget is {budget}")
+        self.needed, self.budget = needed, budget
+
+
 class _Workspace:
     def __init__(self):
         self.sort = ops.SortWorkspace()
@@ -113,14 +122,18 @@ class _BatchedFn(torch.autograd.Function):
         m_dev = meta["m_dev"]
         g = ws.sort.gauss_buf(nv, dev)
         with ops._span("mv_sort", dev):
-            call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), n, V, ptr(vi),
+            tc, to = (ptr(conic), ptr(o)) if meta["tight"] else (None, None)
+            call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V, ptr(vi),
                  lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev), st)
             m = int(m_dev.item()) if capacity is None else int(capacity)
+            if meta["max_instances"] is not None and m > meta["max_instances"]:
+                raise InstanceBudgetExceeded(m, meta["max_instances"])
             ids = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
             rng = torch.empty(lay.total_tiles, 2, dtype=torch.int32, device=dev)
             inst = ws.sort.inst_buf(m, dev)
-            call("gsr_mv_sort_emit", ptr(uv), ptr(radius), n, V, ptr(vi), lay.total_tiles,
-                 lay.tile_w, lay.tile_h, ptr(g), m, ptr(inst), ptr(ids), ptr(rng), st)
+            call("gsr_mv_sort_emit", ptr(uv), ptr(radius), tc, to, n, V, ptr(vi),
+                 lay.total_tiles, lay.tile_w, lay.tile_h, ptr(g), m, ptr(inst), ptr(ids),
+                 ptr(rng), st)
         out = torch.empty(C, lay.total_pixels, device=dev)
         T = torch.empty(lay.total_pixels, device=dev)
         nc = torch.empty(lay.total_pixels, dtype=torch.int32, device=dev)
@@ -200,6 +213,14 @@ class BatchedRender:
         self.ws = _Workspace()
         self._m_dev = None
         self.capacity = None      # instance capacity (graph mode); None = exact
+        # upper bound on the instances (duplicated keys) of one render; a
+        # render that needs more raises InstanceBudgetExceeded before any
+        # instance buffer is allocated (the serving admission control)
+        self.max_instances = None
+        # opacity-aware binning: each splat goes only to the tiles where it
+        # can reach alpha >= 1/255 (same images, fewer instances; ids and
+        # tile ranges then differ from the msplat tile contract)
+        self.tight_tiles = False
         # when True, render() keeps the packed intermediates (uv, depth,
         # conic, radius, ids, tile_range, features, T_final, n_contrib) of
         # all views in self.intermediates
@@ -220,28 +241,14 @@ class BatchedRender:
                           self.tile[0], self.tile[1])
 
     def render(self, cameras, position, opacity, scaling, rotation, shs, layout=None,
-               vi=None, half=None):
+               vi=None, half=None, vf=None):
         """layout / vi / half (the view table and the (W/2, H/2) per view) may
         be passed precomputed, as a captured graph requires (no host data
-        copies inside the step)."""
-        lay = layout or self.layout(cameras)
-        dev = position.device
-        if vi is None:
-            vi = lay.view_int(dev)
-        if self._m_dev is None or self._m_dev.device != dev:
-            self._m_dev = torch.zeros(1, dtype=torch.int32, device=dev)
-        dc, rest = (shs[0], shs[1]) if isinstance(shs, (tuple, list)) else (shs, None)
-        vf = view_floats(cameras)
-        ndc = torch.zeros(lay.V, position.shape[0], 2, device=dev, requires_grad=True)
-        half = torch.tensor([[0.5 * w, 0.5 * h] for w, h in zip(lay.widths, lay.heights)],
-                            device=dev) if half is None else half
-        meta = {"layout": lay, "vi": vi, "flags": self.flags, "degree": self.sh_degree,
-                "bg": self.bg, "batch_depth": self.batch_depth, "ws": self.ws,
-                "capacity": self.capacity, "m_dev": self._m_dev, "half": half,
-                "keep": {} if self.keep_intermediates else None}
-        self.intermediates = meta["keep"]
-        out, T, radius, nc = _BatchedFn.apply(position, scaling, rotation, dc, rest, opacity, vf,
-                                              ndc, meta)
+        copies inside the step); vf, the [V,20] camera table of view_floats,
+        may be passed built on the host and copied once (serving).  Without
+        autograd (torch.no_grad) no ``uv_points`` leaf is allocated."""
+        out, T, radius, ndc, lay = self.render_packed(cameras, position, opacity, scaling,
+                                                      rotation, shs, layout, vi, half, vf)
         views = []
         for v in range(lay.V):
             img = lay.unpack(out, v)
@@ -257,6 +264,36 @@ class BatchedRender:
         rad = radius.view(lay.V, -1)
         return {"views": views, "uv_points": ndc, "visibility": (rad > 0).any(0),
                 "radii": rad.max(0).values, "packed": out, "layout": lay}
+
+    def render_packed(self, cameras, position, opacity, scaling, rotation, shs, layout=None,
+                      vi=None, half=None, vf=None):
+        """-> (out [C, total_pixels], T_final [total_pixels], radius [V*N],
+        uv_points leaf or None, layout): the packed images without the
+        per-view split (layout.unpack(out, v) is view v's [C,H,W])."""
+        lay = layout or self.layout(cameras)
+        dev = position.device
+        if vi is None:
+            vi = lay.view_int(dev)
+        if self._m_dev is None or self._m_dev.device != dev:
+            self._m_dev = torch.zeros(1, dtype=torch.int32, device=dev)
+        dc, rest = (shs[0], shs[1]) if isinstance(shs, (tuple, list)) else (shs, None)
+        if vf is None:
+            vf = view_floats(cameras)
+        if torch.is_grad_enabled():
+            ndc = torch.zeros(lay.V, position.shape[0], 2, device=dev, requires_grad=True)
+            half = torch.tensor([[0.5 * w, 0.5 * h] for w, h in zip(lay.widths, lay.heights)],
+                                device=dev) if half is None else half
+        else:
+            ndc = None
+        meta = {"layout": lay, "vi": vi, "flags": self.flags, "degree": self.sh_degree,
+                "bg": self.bg, "batch_depth": self.batch_depth, "ws": self.ws,
+                "capacity": self.capacity, "m_dev": self._m_dev, "half": half,
+                "max_instances": self.max_instances, "tight": self.tight_tiles,
+                "keep": {} if self.keep_intermediates else None}
+        self.intermediates = meta["keep"]
+        out, T, radius, nc = _BatchedFn.apply(position, scaling, rotation, dc, rest, opacity, vf,
+                                              ndc, meta)
+        return out, T, radius, ndc, lay
 
     def instances(self) -> int:
         """Device instance count of the last forward (host read)."""

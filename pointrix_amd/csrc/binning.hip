@@ -276,24 +276,50 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 // --------------------------------------------------------------------------
 // Binning kernels
 // --------------------------------------------------------------------------
-// Tile grid of the items being binned.  One view: (gx, gy) with vi ==
-// nullptr.  Several views: item j belongs to view j / npv, whose grid and
-// first global tile come from the view table vi (int32 [V][8]: W, H, gx, gy,
-// tile_off, pix_off, -, -); emitted tile ids are global (tile_off + local).
+// Tile grid of the items being binned.  One view: (gx, gy) and image size
+// (W, H) with vi == nullptr.  Several views: item j belongs to view j / npv,
+// whose grid, size and first global tile come from the view table vi (int32
+// [V][8]: W, H, gx, gy, tile_off, pix_off, -, -); emitted tile ids are global
+// (tile_off + local).  With conic != nullptr ("tight" binning) a splat is
+// emitted only into the tiles of its rectangle that tile_reachable accepts
+// (opacity of item j at opac[j % npv], or opac[j] for one view).
 struct Grid {
   int gx, gy, tw, th;
   const int32_t *vi;
   uint32_t npv;
+  int W, H;
+  const float *conic;
+  const float *opac;
 };
 
 struct ItemGrid {
-  int gx, gy, toff;
+  int gx, gy, toff, W, H;
 };
 
 __device__ __forceinline__ ItemGrid item_grid(const Grid &g, uint32_t j) {
-  if (g.vi == nullptr) return ItemGrid{g.gx, g.gy, 0};
+  if (g.vi == nullptr) return ItemGrid{g.gx, g.gy, 0, g.W, g.H};
   const int32_t *row = g.vi + 8 * (j / g.npv);
-  return ItemGrid{row[2], row[3], row[4]};
+  return ItemGrid{row[2], row[3], row[4], row[0], row[1]};
+}
+
+// Splat parameters of the tight tile test.
+struct TightSplat {
+  float u, v, a, b, c, o;
+};
+
+__device__ __forceinline__ TightSplat tight_splat(const Grid &g, const float *uv, uint32_t j) {
+  const uint32_t oi = g.vi == nullptr ? j : j % g.npv;
+  return TightSplat{uv[2 * (int64_t)j], uv[2 * (int64_t)j + 1], g.conic[3 * (int64_t)j],
+                    g.conic[3 * (int64_t)j + 1], g.conic[3 * (int64_t)j + 2], g.opac[oi]};
+}
+
+// tile (tx, ty) of a view of size W x H: pixel-centre box, then the test
+__device__ __forceinline__ bool tight_tile(const TightSplat &s, int tx, int ty, int tw, int th,
+                                           int W, int H) {
+  const int px0 = tx * tw, py0 = ty * th;
+  const int px1 = min(px0 + tw, W) - 1, py1 = min(py0 + th, H) - 1;
+  return tile_reachable(s.u, s.v, s.a, s.b, s.c, s.o, (float)px0, (float)px1, (float)py0,
+                        (float)py1);
 }
 
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
@@ -307,7 +333,13 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
   if (r > 0) {
     const ItemGrid ig = item_grid(g, i);
     const Rect q = tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
-    c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
+    if (g.conic == nullptr) {
+      c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
+    } else {
+      const TightSplat sp = tight_splat(g, uv, i);
+      for (int ty = q.y0; ty < q.y1; ++ty)
+        for (int tx = q.x0; tx < q.x1; ++tx) c += tight_tile(sp, tx, ty, g.tw, g.th, ig.W, ig.H);
+    }
   }
   cnt[i] = c;
 }
@@ -383,6 +415,85 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
       const int col = k - row * jw;
       tkeys[joff + k] = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
       tvals[joff + k] = jid;
+    }
+  }
+}
+
+// Tight emission: as emit_kernel, but for each splat the wave tests the
+// tiles of its rectangle 64 at a time and writes the accepted ones, in
+// row-major order, compacted with a ballot (count_kernel counted exactly the
+// same tests, so every splat fills [offset, offset + count)).
+__global__ __launch_bounds__(kThreads) void emit_tight_kernel(
+    const float *__restrict__ uv, const int32_t *__restrict__ radius,
+    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
+    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t cap_inst,
+    uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
+  const uint32_t nv = count_of(nv_ptr, cap);
+  const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
+  if (wave_base >= nv) return;  // uniform per wave
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t s = wave_base + lane;
+  int id = 0, x0 = 0, y0 = 0, w = 1, area = 0, gxi = 0, toff = 0, Wv = 0, Hv = 0;
+  uint32_t off = 0;
+  TightSplat sp{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (s < nv) {
+    id = sorted_ids[s];
+    const ItemGrid ig = item_grid(g, (uint32_t)id);
+    gxi = ig.gx;
+    toff = ig.toff;
+    Wv = ig.W;
+    Hv = ig.H;
+    const Rect q = tile_rect(uv[2 * (int64_t)id], uv[2 * (int64_t)id + 1], radius[id], ig.gx,
+                             ig.gy, g.tw, g.th);
+    x0 = q.x0;
+    y0 = q.y0;
+    w = max(q.x1 - q.x0, 1);
+    area = (q.x1 - q.x0) * (q.y1 - q.y0);
+    off = offsets[s];
+    sp = tight_splat(g, uv, (uint32_t)id);
+  }
+  const int nsplat = min(64, (int)(nv - wave_base));
+  for (int j = 0; j < nsplat; ++j) {
+    const int jid = __builtin_amdgcn_readlane(id, j);
+    const int jx0 = __builtin_amdgcn_readlane(x0, j);
+    const int jy0 = __builtin_amdgcn_readlane(y0, j);
+    const int jw = __builtin_amdgcn_readlane(w, j);
+    const int jarea = __builtin_amdgcn_readlane(area, j);
+    const int jgx = __builtin_amdgcn_readlane(gxi, j);
+    const int jtoff = __builtin_amdgcn_readlane(toff, j);
+    const int jW = __builtin_amdgcn_readlane(Wv, j);
+    const int jH = __builtin_amdgcn_readlane(Hv, j);
+    const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
+    TightSplat js;
+    js.u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.u), j));
+    js.v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.v), j));
+    js.a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.a), j));
+    js.b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.b), j));
+    js.c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.c), j));
+    js.o = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.o), j));
+    const float inv_w = 1.0f / (float)jw;
+    uint32_t run = 0;
+    for (int k0 = 0; k0 < jarea; k0 += 64) {
+      const int k = k0 + lane;
+      bool pass = false;
+      uint32_t key = 0;
+      if (k < jarea) {
+        const int row = (int)(((float)k + 0.5f) * inv_w);
+        const int col = k - row * jw;
+        pass = tight_tile(js, jx0 + col, jy0 + row, g.tw, g.th, jW, jH);
+        key = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
+      }
+      const uint64_t bal = __ballot(pass);
+      if (pass) {
+        // capacity guard as in emit_kernel
+        const uint32_t pos = joff + run + (uint32_t)__popcll(bal & lt);
+        if (pos < cap_inst) {
+          tkeys[pos] = key;
+          tvals[pos] = jid;
+        }
+      }
+      run += (uint32_t)__popcll(bal);
     }
   }
 }
@@ -557,9 +668,13 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;
-  emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, g.sb.valA, g.offs,
-                                                           g.scalars, (uint32_t)n, gr,
-                                                           (uint32_t)m, ib.keyA, v_first);
+  if (gr.conic == nullptr)
+    emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, g.sb.valA, g.offs,
+                                                             g.scalars, (uint32_t)n, gr,
+                                                             (uint32_t)m, ib.keyA, v_first);
+  else
+    emit_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(
+        uv, radius, g.sb.valA, g.offs, g.scalars, (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
   run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
@@ -572,46 +687,69 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
 
 extern "C" {
 
+int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *radius,
+                         const float *conic, const float *opacity, int64_t n, int W, int H,
+                         int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
+              "sort_count: bad sizes");
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "sort_count: conic without opacity");
+  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0,
+                W, H, conic, opacity};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
+}
+
+int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *conic,
+                        const float *opacity, int64_t n, int W, int H, int tile_w, int tile_h,
+                        void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
+                        int32_t *tile_range, void *stream) {
+  GSR_REQUIRE(n >= 0 && m >= 0 && m < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 &&
+                  tile_h > 0,
+              "sort_emit: bad sizes");
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "sort_emit: conic without opacity");
+  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+  const Grid gr{gx, gy, tile_w, tile_h, nullptr, 0, W, H, conic, opacity};
+  return sort_emit_impl(uv, radius, n, gr, gx * gy, ws_gauss, m, ws_inst, ids_out, tile_range,
+                        (hipStream_t)stream);
+}
+
 int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,
                    int W, int H, int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
                    void *stream) {
-  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
-              "sort_count: bad sizes");
-  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0};
-  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
+  return gsr_sort_count_tight(uv, depth, radius, nullptr, nullptr, n, W, H, tile_w, tile_h,
+                              ws_gauss, m_out, stream);
 }
 
 int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int H, int tile_w,
                   int tile_h, void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
                   int32_t *tile_range, void *stream) {
-  GSR_REQUIRE(n >= 0 && m >= 0 && m < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 &&
-                  tile_h > 0,
-              "sort_emit: bad sizes");
-  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
-  const Grid gr{gx, gy, tile_w, tile_h, nullptr, 0};
-  return sort_emit_impl(uv, radius, n, gr, gx * gy, ws_gauss, m, ws_inst, ids_out, tile_range,
-                        (hipStream_t)stream);
+  return gsr_sort_emit_tight(uv, radius, nullptr, nullptr, n, W, H, tile_w, tile_h, ws_gauss, m,
+                             ws_inst, ids_out, tile_range, stream);
 }
 
 int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
-                      int64_t n_per_view, int V, const int32_t *view_int, int tile_w, int tile_h,
-                      void *ws_gauss, int32_t *m_out, void *stream) {
+                      const float *conic, const float *opacity, int64_t n_per_view, int V,
+                      const int32_t *view_int, int tile_w, int tile_h, void *ws_gauss,
+                      int32_t *m_out, void *stream) {
   const int64_t n = n_per_view * (int64_t)V;
   GSR_REQUIRE(V >= 1 && n_per_view >= 0 && n < (int64_t)INT32_MAX && tile_w > 0 && tile_h > 0,
               "mv_sort_count: bad sizes");
-  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1)};
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "mv_sort_count: conic without opacity");
+  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1),
+                0, 0, conic, opacity};
   return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
 }
 
-int gsr_mv_sort_emit(const float *uv, const int32_t *radius, int64_t n_per_view, int V,
-                     const int32_t *view_int, int total_tiles, int tile_w, int tile_h,
-                     void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
-                     int32_t *tile_range, void *stream) {
+int gsr_mv_sort_emit(const float *uv, const int32_t *radius, const float *conic,
+                     const float *opacity, int64_t n_per_view, int V, const int32_t *view_int,
+                     int total_tiles, int tile_w, int tile_h, void *ws_gauss, int64_t m,
+                     void *ws_inst, int32_t *ids_out, int32_t *tile_range, void *stream) {
   const int64_t n = n_per_view * (int64_t)V;
   GSR_REQUIRE(V >= 1 && n_per_view >= 0 && n < (int64_t)INT32_MAX && m >= 0 &&
                   m < (int64_t)INT32_MAX && total_tiles > 0,
               "mv_sort_emit: bad sizes");
-  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1)};
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "mv_sort_emit: conic without opacity");
+  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1),
+                0, 0, conic, opacity};
   return sort_emit_impl(uv, radius, n, gr, total_tiles, ws_gauss, m, ws_inst, ids_out,
                         tile_range, (hipStream_t)stream);
 }

@@ -63,6 +63,39 @@ __device__ __forceinline__ Rect tile_rect(float u, float v, int r, int gx, int g
   return q;
 }
 
+// Opacity-aware tile test.  Whether a splat centred at (u, v) with conic
+// (a, b, c) and opacity o can reach alpha >= kAlphaMin at some pixel centre of
+// the box [x0, x1] x [y0, y1] (inclusive pixel coordinates).  Alpha at pixel p
+// is o * exp(-q/2) with q = a dx^2 + 2 b dx dy + c dy^2, d = (u, v) - p, so the
+// test takes the minimum of the (convex) q over the continuous box -- zero if
+// the centre is inside, else the least of the four edge minima -- lowers it
+// by a margin (1e-3 + 1e-4 * S, S bounding |a| dx^2 + 2|b||dx dy| + |c| dy^2
+// on the box; far above the rounding of the blend's FP32 evaluation of q and
+// of exp) and compares alpha there with kAlphaMin.  Never false where a pixel
+// of the box reaches the threshold, so dropping the tiles it rejects leaves
+// every blended pixel unchanged.
+__device__ __forceinline__ float quad_edge_min(float a, float b, float c, float X, float y0,
+                                               float y1) {
+  // min over y in [y0, y1] of a X^2 + 2 b X y + c y^2 (c > 0)
+  const float y = fminf(fmaxf(-(b * X) / c, y0), y1);
+  return a * X * X + 2.0f * b * X * y + c * y * y;
+}
+
+__device__ __forceinline__ bool tile_reachable(float u, float v, float a, float b, float c,
+                                               float o, float x0, float x1, float y0, float y1) {
+  if (!(a > 0.0f) || !(c > 0.0f)) return true;
+  const float dx0 = u - x1, dx1 = u - x0, dy0 = v - y1, dy1 = v - y0;
+  if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return true;
+  float m = quad_edge_min(a, b, c, dx0, dy0, dy1);
+  m = fminf(m, quad_edge_min(a, b, c, dx1, dy0, dy1));
+  m = fminf(m, quad_edge_min(c, b, a, dy0, dx0, dx1));
+  m = fminf(m, quad_edge_min(c, b, a, dy1, dx0, dx1));
+  const float DX = fmaxf(fabsf(dx0), fabsf(dx1)), DY = fmaxf(fabsf(dy0), fabsf(dy1));
+  const float S = fabsf(a) * DX * DX + 2.0f * fabsf(b) * DX * DY + fabsf(c) * DY * DY;
+  const float t = fmaxf(m - (1e-3f + 1e-4f * S), 0.0f);
+  return o * exp_neg(-0.5f * t) >= kAlphaMin;
+}
+
 // ---------------------------------------------------------------------------
 // wave64 helpers
 // ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <stdarg.h>
 #include <stdint.h>
 
+#include "gsr.h"
 #include "gsr_common.h"
 
 void gsr_set_error(const char *fmt, ...);

@@ -271,6 +271,12 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     ones tile_range points at, are meaningful).  ``tiles`` is accepted for msplat
     signature compatibility; counts are recomputed from (uv, radius), which is
     what ewa_project's ``tiles`` holds.
+
+    With ``conic`` [N,3] and ``opacity`` [N] (an extension, no msplat
+    counterpart) the binning is opacity-aware: a splat goes only to the tiles
+    of its rectangle where it can reach alpha >= 1/255, which blends to the
+    same images with fewer instances (each tile's list is then a subsequence
+    of the exact one).
     """
     _dev_check(uv, depth, radius)
     ws = workspace or _default_ws
@@ -283,9 +289,15 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     gx, gy = (W + tile_w - 1) // tile_w, (H + tile_h - 1) // tile_h
     m_dev = ws.count_buf(dev)
     g = ws.gauss_buf(n, dev)
+    if (conic is None) != (opacity is None):
+        raise ValueError("sort_gaussian: pass conic and opacity together")
+    if conic is not None:
+        conic = _f32(conic)
+        opacity = _f32(opacity).reshape(-1)
+    tc, to = (None, None) if conic is None else (ptr(conic), ptr(opacity))
     with _span("sort_count", dev):
-        call("gsr_sort_count", ptr(u), ptr(d), ptr(r), n, int(W), int(H), tile_w, tile_h,
-             ptr(g), ptr(m_dev), st)
+        call("gsr_sort_count_tight", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H), tile_w,
+             tile_h, ptr(g), ptr(m_dev), st)
     if ws.capacity is None:
         m = int(m_dev.item())
         global last_instance_count
@@ -296,8 +308,8 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     tile_range = torch.empty(gx * gy, 2, dtype=torch.int32, device=dev)
     inst = ws.inst_buf(m, dev)
     with _span("sort_emit", dev):
-        call("gsr_sort_emit", ptr(u), ptr(r), n, int(W), int(H), tile_w, tile_h, ptr(g), m,
-             ptr(inst), ptr(ids), ptr(tile_range), st)
+        call("gsr_sort_emit_tight", ptr(u), ptr(r), tc, to, n, int(W), int(H), tile_w, tile_h,
+             ptr(g), m, ptr(inst), ptr(ids), ptr(tile_range), st)
     return ids, tile_range
 
 

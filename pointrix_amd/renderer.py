@@ -108,6 +108,10 @@ class MsplatRender:
         # one fused HIP kernel for direction/SH/projection/covariance/EWA
         # (False: the six msplat-style ops one after another)
         fused: bool = True
+        # opacity-aware binning in the fused path: same images, fewer
+        # (splat, tile) instances; off by default because the sorted ids and
+        # tile ranges then differ from the msplat tile contract
+        tight_tiles: bool = False
 
     def __init__(self, cfg=None, white_bg=False, device="cuda", **kwargs):
         if cfg is None:
@@ -195,8 +199,10 @@ class MsplatRender:
         if normals is not None:
             feat = torch.cat([feat, normals.to(feat.dtype)], -1)
             names.append(("normal", normals.shape[-1]))
+        tight = (conic.detach(), opacity.detach()) if self.cfg.tight_tiles else (None, None)
         ids, tile_range = ops.sort_gaussian(uv, depth, width, height, radius, tiles,
-                                            workspace=self.workspace)
+                                            workspace=self.workspace, conic=tight[0],
+                                            opacity=tight[1])
         ndc = torch.zeros_like(uv, requires_grad=True)
         ndc.retain_grad()
         rendered, T_final, n_contrib = ops.alpha_blending_ex(uv, conic, opacity, feat, ids,
