@@ -230,6 +230,34 @@ int gsr_l1_loss_fwd(const float *x, const float *y, int64_t n, float *partial, f
 int gsr_l1_loss_bwd(const float *x, const float *y, int64_t n, const float *g_out, float *dx,
                     void *stream);
 
+/* ---- fused L1 + SSIM (pointrix/model/loss.py:27, :73-116; combined as in
+ * base_model.py:112-124) over `planes` planes of W x H ([B,C,H,W]: B*C planes).
+ * out[0] = mean SSIM (11-tap Gaussian window, sigma 1.5, zero padding),
+ * out[1] = mean |x - y|; partial[gsr_ssim_partials(planes, W, H)] caller
+ * scratch; fixed-order sums, no atomics, no memset (graph-safe).  Backward:
+ * dx = d(out)/dx weighted by g[0] (for out[0]) and g[1] (for out[1]), both
+ * device scalars. */
+int gsr_ssim_partials(int planes, int W, int H);
+int gsr_l1_ssim_fwd(const float *x, const float *y, int planes, int W, int H, float *partial,
+                    float *out, void *stream);
+int gsr_l1_ssim_bwd(const float *x, const float *y, int planes, int W, int H, const float *g,
+                    float *dx, void *stream);
+
+/* ---- densification statistics (controller/gs.py:318-333, 259-284) -------
+ * For every splat with visibility[i] != 0: g = (sum over the V views of
+ * grad[v*view_stride + i*row_stride + {0,1}]) * (sx, sy) (the views summed in
+ * order, then scaled: normalize_grad's (W/2, H/2), or (1, 1));
+ * grad_accum[i] += |g|; acc_steps[i] += 1;
+ * max_radii[i] = max(max_radii[i], radii[i]).  Other splats are untouched. */
+int gsr_densify_stats(const float *grad, int64_t view_stride, int64_t row_stride, int V,
+                      int64_t n, const uint8_t *visibility, const int32_t *radii, float sx,
+                      float sy, float *grad_accum, float *acc_steps, float *max_radii,
+                      void *stream);
+/* uv_points.grad [n,2] = the (du, dv) of the blend backward's records
+ * (d_splat[i*ld + 0..1]) scaled by (sx, sy) = (W/2, H/2) (msplat.py:141-145). */
+int gsr_uv_grad_scale(const float *d_splat, int ld, int64_t n, float sx, float sy, float *out,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

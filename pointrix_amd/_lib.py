@@ -55,6 +55,11 @@ SIGNATURES = {
                                        _I, _I, _P, _P, _P, _P, _P]),
     "gsr_mv_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I64, _I, _P, _I, _I64, _I,
                                        _I, _I, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "gsr_ssim_partials": (_I, [_I, _I, _I]),
+    "gsr_l1_ssim_fwd": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
+    "gsr_l1_ssim_bwd": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
+    "gsr_densify_stats": (_I, [_P, _I64, _I64, _I, _I64, _P, _P, _F, _F, _P, _P, _P, _P]),
+    "gsr_uv_grad_scale": (_I, [_P, _I, _I64, _F, _F, _P, _P]),
     "gsr_l1_loss_partials": (_I, [_I64]),
     "gsr_l1_loss_fwd": (_I, [_P, _P, _I64, _P, _P, _P]),
     "gsr_l1_loss_bwd": (_I, [_P, _P, _I64, _P, _P, _P]),

@@ -356,7 +356,9 @@ class _AlphaBlending(torch.autograd.Function):
         d_u, d_c, d_o, d_f = rec[:, 0:2], rec[:, 2:5], rec[:, 5], rec[:, 6:6 + C]
         d_ndc = None
         if ctx.needs_input_grad[9]:
-            d_ndc = torch.stack((d_u[:, 0] * (0.5 * W), d_u[:, 1] * (0.5 * H)), 1)
+            d_ndc = torch.empty(n, 2, device=dev)
+            call("gsr_uv_grad_scale", ptr(rec), ld, n, 0.5 * W, 0.5 * H, ptr(d_ndc),
+                 stream_of(u))
         return (d_u, d_c, d_o.reshape(op_shape), d_f, None, None, None, None, None, d_ndc, None,
                 None)
 
@@ -569,3 +571,49 @@ class _L1Loss(torch.autograd.Function):
 def l1_loss(x, y):
     """mean |x - y| as one deterministic, graph-capturable reduction."""
     return _L1Loss.apply(x, y)
+
+
+# ---------------------------------------------------------------------------
+# fused L1 + SSIM loss (pointrix/model/loss.py:27, :73-116; base_model.py:112-124)
+# ---------------------------------------------------------------------------
+class _L1Ssim(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, y):
+        xc, yc = _f32(x), _f32(y)
+        H, W = xc.shape[-2], xc.shape[-1]
+        planes = xc.numel() // (H * W)
+        dev = xc.device
+        part = torch.empty(int(_lib.load().gsr_ssim_partials(planes, W, H)), device=dev)
+        out = torch.empty(2, device=dev)
+        call("gsr_l1_ssim_fwd", ptr(xc), ptr(yc), planes, W, H, ptr(part), ptr(out),
+             stream_of(xc))
+        ctx.save_for_backward(xc, yc)
+        ctx.dims = (planes, W, H, x.shape)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        xc, yc = ctx.saved_tensors
+        planes, W, H, shape = ctx.dims
+        dx = torch.empty_like(xc)
+        call("gsr_l1_ssim_bwd", ptr(xc), ptr(yc), planes, W, H, ptr(_f32(g)), ptr(dx),
+             stream_of(xc))
+        return dx.view(shape), None
+
+
+def l1_ssim(pred, gt):
+    """-> tensor [2] = (mean SSIM, mean |pred - gt|), differentiable in pred;
+    pred and gt [..., H, W] (e.g. [B,3,H,W]) on the GPU."""
+    if pred.shape != gt.shape:
+        raise ValueError("l1_ssim: shapes differ")
+    _dev_check(pred, gt)
+    return _L1Ssim.apply(pred, gt)
+
+
+def l1_ssim_loss(pred, gt, lambda_ssim=0.2):
+    """BaseModel.get_loss_dict (base_model.py:112-124): loss =
+    (1 - lambda) * L1 + lambda * (1 - SSIM); returns the same dict."""
+    s = l1_ssim(pred, gt)
+    ssim_loss = 1.0 - s[0]
+    loss = (1.0 - lambda_ssim) * s[1] + lambda_ssim * ssim_loss
+    return {"loss": loss, "L1_loss": s[1], "ssim_loss": ssim_loss}
