@@ -30,6 +30,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+# workload, each scaled by the counter's reading for a copy of known size
+# (fetch x2.0, write x1.0); summary in profiles/alpha_blending_bwd_full3m_pmc.md
+PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 98051 * 1024 * 2 + 89086 * 1024,
+                     "alpha_blending_fwd": 75996 * 1024 * 2 + 51523 * 1024,
+                     "preprocess_bwd": 304476 * 1024 * 2 + 680747 * 1024,
+                     "preprocess_fwd": 228052 * 1024 * 2 + 175782 * 1024}
+
 
 def quat_to_rotmat(q):
     from pointrix_amd.renderer import quat_to_rotmat as q2r
@@ -212,6 +220,24 @@ def cpu_baseline(workload_fn, div, seed=0):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+def time_tight(renderer, step, steps):
+    """The step with MsplatRender's tight_tiles on (opacity-aware binning)."""
+    from pointrix_amd import ops
+    renderer.cfg.tight_tiles = True
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    m = ops.last_instance_count
+    renderer.cfg.tight_tiles = False
+    print(f"[bench] tight tiles: {ms:.3f} ms/step, {m} instances", file=sys.stderr, flush=True)
+    return {"ms_per_step": round(ms, 4), "instances": m}
+
+
 def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
     """1M splats (SH 3), 16 cameras of uneven resolution,
     summed L1 over views, fwd+bwd.  Plain = one view per launch
@@ -379,6 +405,8 @@ def main():
     ap.add_argument("--cpu-div", type=int, default=128)
     ap.add_argument("--batched", type=int, default=1, help="also run the 16-view batched case")
     ap.add_argument("--serve", type=int, default=1, help="also run the serving trace")
+    ap.add_argument("--tight", type=int, default=1,
+                    help="also time the step with opacity-aware binning")
     ap.add_argument("--graph", type=int, default=1,
                     help="also time the step captured as one HIP graph (value = the faster "
                          "of eager and graph, the graph only if its replays match eager)")
@@ -463,26 +491,18 @@ def main():
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches"])
     dom_bytes = ab.get(dom, ab["binning"] if dom.startswith("sort") else None)
     achieved = dom_bytes / (per_kernel[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = PMC_TRAFFIC_BYTES.get(dom) if args.workload == "full3m" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None}
+                "traffic": traffic, "algorithmic_bytes": int(dom_bytes),
+                "traffic_source": "profiles/alpha_blending_bwd_full3m_pmc.md" if traffic else None}
     print(f"[bench] per-kernel: {json.dumps(per_kernel)}", file=sys.stderr, flush=True)
 
     # the same step with opacity-aware binning (same images, fewer instances;
     # not the msplat tile contract, so reported beside the headline only)
-    renderer.cfg.tight_tiles = True
-    for _ in range(2):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms_tight = (time.perf_counter() - t0) * 1000 / args.steps
-    m_tight = ops.last_instance_count
-    renderer.cfg.tight_tiles = False
-    print(f"[bench] tight tiles: {ms_tight:.3f} ms/step, {m_tight} instances",
-          file=sys.stderr, flush=True)
+    tight = None
+    if args.tight:
+        tight = time_tight(renderer, step, args.steps)
 
     graph = None
     if args.graph:
@@ -525,7 +545,7 @@ def main():
                    "splats": N, "width": W, "height": H, "visible": Nv, "instances": M,
                    "mpixel_per_s": round(P / 1e6 * 1000.0 / ms, 2),
                    "eager_ms_per_step": round(ms_eager, 4), "graph": graph,
-                   "tight_tiles": {"ms_per_step": round(ms_tight, 4), "instances": m_tight},
+                   "tight_tiles": tight,
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,

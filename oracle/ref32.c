@@ -284,52 +284,90 @@ int64_t ref_sort(const float *uv, const float *depth, const int32_t *radius, int
 }
 
 /* ---- opacity-aware ("tight") binning -----------------------------------
- * Extension of the build with no msplat counterpart.  A splat is kept in a
- * tile of its rectangle when o * exp(-q/2) >= 1/255 can hold at some pixel
- * centre of the tile, q = a dx^2 + 2 b dx dy + c dy^2 with d = uv - pixel:
- * q's minimum over the tile's pixel-centre box (0 with the centre inside,
- * else the least of the four edge minima, each edge a 1-D quadratic clamped
- * to the edge), lowered by the margin 1e-3 + 1e-4 * S (S bounds the sum of
- * the terms' magnitudes over the box), decides.  Same operations, in the same
+ * Extension of the build with no msplat counterpart.  A splat reaches
+ * alpha >= 1/255 at pixel p only if q = a dx^2 + 2 b dx dy + c dy^2 <=
+ * tau = 2 ln(o * 255), d = uv - p (none at all when o < 1/255).  A tile of
+ * its rectangle is kept when q's minimum over the tile's pixel-centre box (0
+ * with the centre inside, else the least edge minimum, each edge's
+ * minimiser -(b/c) X or -(b/a) Y clamped to the edge), lowered by the margin
+ * 1e-3 + 1e-4 * S (S bounds the terms' magnitudes on the box), is <= tau.
+ * ln is a frexp + atanh-series evaluation.  Same operations, in the same
  * order, as the kernels; the blend is checked to give identical images. */
-static float quad_edge_min(float a, float b, float c, float X, float y0, float y1) {
-  const float y = fminf(fmaxf(-(b * X) / c, y0), y1);
-  return a * X * X + 2.0f * b * X * y + c * y * y;
+static float log_pos(float x) {
+  int e;
+  float m = frexpf(x, &e), s, s2, p;
+  if (m < 0.70710678f) {
+    m = m * 2.0f;
+    e = e - 1;
+  }
+  s = (m - 1.0f) / (m + 1.0f);
+  s2 = s * s;
+  p = 1.0f / 9.0f;
+  p = fmaf(p, s2, 1.0f / 7.0f);
+  p = fmaf(p, s2, 1.0f / 5.0f);
+  p = fmaf(p, s2, 1.0f / 3.0f);
+  p = fmaf(p, s2, 1.0f);
+  return fmaf(2.0f * s, p, (float)e * 0.69314718f);
 }
 
-static int tile_reachable(float u, float v, float a, float b, float c, float o, float x0,
-                          float x1, float y0, float y1) {
+float ref_log_pos(float x) { return log_pos(x); }
+
+typedef struct {
+  float u, v, a, b, c, rbc, rba, tau;
+  int reach;
+} Tight;
+
+static Tight tight_setup(float u, float v, float a, float b, float c, float o) {
   const float alpha_min = 1.0f / 255.0f;
-  float dx0, dx1, dy0, dy1, m, DX, DY, S, t;
-  if (!(a > 0.0f) || !(c > 0.0f)) return 1;
-  dx0 = u - x1;
-  dx1 = u - x0;
-  dy0 = v - y1;
-  dy1 = v - y0;
+  Tight t;
+  const int pd = a > 0.0f && c > 0.0f;
+  t.u = u;
+  t.v = v;
+  t.a = a;
+  t.b = b;
+  t.c = c;
+  t.reach = o >= alpha_min;
+  t.tau = t.reach ? 2.0f * log_pos(o / alpha_min) : 0.0f;
+  t.rbc = pd ? b / c : 0.0f;
+  t.rba = pd ? b / a : 0.0f;
+  return t;
+}
+
+static int tile_reachable(const Tight *t, float x0, float x1, float y0, float y1) {
+  float dx0, dx1, dy0, dy1, m, ya, yb, xa, xb, DX, DY, S;
+  if (!(t->a > 0.0f) || !(t->c > 0.0f)) return 1;
+  dx0 = t->u - x1;
+  dx1 = t->u - x0;
+  dy0 = t->v - y1;
+  dy1 = t->v - y0;
   if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return 1;
-  m = quad_edge_min(a, b, c, dx0, dy0, dy1);
-  m = fminf(m, quad_edge_min(a, b, c, dx1, dy0, dy1));
-  m = fminf(m, quad_edge_min(c, b, a, dy0, dx0, dx1));
-  m = fminf(m, quad_edge_min(c, b, a, dy1, dx0, dx1));
+  ya = fminf(fmaxf(-(t->rbc * dx0), dy0), dy1);
+  yb = fminf(fmaxf(-(t->rbc * dx1), dy0), dy1);
+  xa = fminf(fmaxf(-(t->rba * dy0), dx0), dx1);
+  xb = fminf(fmaxf(-(t->rba * dy1), dx0), dx1);
+  m = t->a * dx0 * dx0 + 2.0f * t->b * dx0 * ya + t->c * ya * ya;
+  m = fminf(m, t->a * dx1 * dx1 + 2.0f * t->b * dx1 * yb + t->c * yb * yb);
+  m = fminf(m, t->c * dy0 * dy0 + 2.0f * t->b * dy0 * xa + t->a * xa * xa);
+  m = fminf(m, t->c * dy1 * dy1 + 2.0f * t->b * dy1 * xb + t->a * xb * xb);
   DX = fmaxf(fabsf(dx0), fabsf(dx1));
   DY = fmaxf(fabsf(dy0), fabsf(dy1));
-  S = fabsf(a) * DX * DX + 2.0f * fabsf(b) * DX * DY + fabsf(c) * DY * DY;
-  t = fmaxf(m - (1e-3f + 1e-4f * S), 0.0f);
-  return o * exp_neg(-0.5f * t) >= alpha_min;
+  S = fabsf(t->a) * DX * DX + 2.0f * fabsf(t->b) * DX * DY + fabsf(t->c) * DY * DY;
+  return m - (1e-3f + 1e-4f * S) <= t->tau;
 }
 
 int ref_tile_reachable(float u, float v, float a, float b, float c, float o, float x0, float x1,
                        float y0, float y1) {
-  return tile_reachable(u, v, a, b, c, o, x0, x1, y0, y1);
+  const Tight t = tight_setup(u, v, a, b, c, o);
+  return t.reach && tile_reachable(&t, x0, x1, y0, y1);
 }
 
 static int tile_keep(const float *uv, const float *conic, const float *opacity, int64_t i,
                      int tx, int ty, int tw, int th, int W, int H) {
   const int px0 = tx * tw, py0 = ty * th;
   const int px1 = (px0 + tw < W ? px0 + tw : W) - 1, py1 = (py0 + th < H ? py0 + th : H) - 1;
-  return tile_reachable(uv[2 * i], uv[2 * i + 1], conic[3 * i], conic[3 * i + 1],
-                        conic[3 * i + 2], opacity[i], (float)px0, (float)px1, (float)py0,
-                        (float)py1);
+  const Tight t = tight_setup(uv[2 * i], uv[2 * i + 1], conic[3 * i], conic[3 * i + 1],
+                              conic[3 * i + 2], opacity[i]);
+  return t.reach && tile_reachable(&t, (float)px0, (float)px1, (float)py0, (float)py1);
 }
 
 /* as ref_sort, keeping only the tiles tile_keep accepts; returns M */

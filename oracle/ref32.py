@@ -32,6 +32,8 @@ def lib():
         L.ref_sort_count.argtypes = [P, P, I64, I, I, I, I]
         L.ref_sort.restype = I64
         L.ref_sort.argtypes = [P, P, P, I64, I, I, I, I, P, P]
+        L.ref_log_pos.restype = F
+        L.ref_log_pos.argtypes = [F]
         L.ref_tile_reachable.restype = I
         L.ref_tile_reachable.argtypes = [F] * 10
         L.ref_sort_tight.restype = I64
@@ -109,6 +111,10 @@ def sort_gaussian(uv, depth, W, H, radius, tile_w=16, tile_h=16):
     rng = np.zeros((gx * gy, 2), np.int32)
     lib().ref_sort(_p(u), _p(d), _p(r), n, W, H, tile_w, tile_h, _p(ids), _p(rng))
     return ids[:m], rng
+
+
+def log_pos(x: float) -> float:
+    return float(lib().ref_log_pos(float(x)))
 
 
 def tile_reachable(u, v, a, b, c, o, x0, x1, y0, y1) -> bool:

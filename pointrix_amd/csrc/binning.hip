@@ -302,24 +302,98 @@ __device__ __forceinline__ ItemGrid item_grid(const Grid &g, uint32_t j) {
   return ItemGrid{row[2], row[3], row[4], row[0], row[1]};
 }
 
-// Splat parameters of the tight tile test.
-struct TightSplat {
-  float u, v, a, b, c, o;
+// ---- tight binning: a wave works through the flattened (splat, tile)
+// candidates of its 64 splats, 64 candidates at a time.  Lane l holds splat l
+// (rectangle, test parameters) and the exclusive prefix P_l of the
+// rectangle areas; candidate c belongs to the last lane with P <= c (found by
+// a 6-step binary search over shuffles), whose parameters are then shuffled
+// in.  Candidates are ordered by splat, then row-major over the rectangle.
+struct TLane {
+  TightSplat t;
+  int x0, y0, w, area, W, H;
 };
 
-__device__ __forceinline__ TightSplat tight_splat(const Grid &g, const float *uv, uint32_t j) {
+__device__ __forceinline__ TLane tight_lane(const Grid &g, const float *uv,
+                                            const int32_t *radius, uint32_t j, bool valid) {
+  TLane L;
+  L.t = TightSplat{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  L.x0 = L.y0 = L.area = L.W = L.H = 0;
+  L.w = 1;
+  if (!valid) return L;
+  const int r = radius[j];
+  if (r <= 0) return L;
+  const ItemGrid ig = item_grid(g, j);
+  const float u = uv[2 * (int64_t)j], v = uv[2 * (int64_t)j + 1];
+  const Rect q = tile_rect(u, v, r, ig.gx, ig.gy, g.tw, g.th);
   const uint32_t oi = g.vi == nullptr ? j : j % g.npv;
-  return TightSplat{uv[2 * (int64_t)j], uv[2 * (int64_t)j + 1], g.conic[3 * (int64_t)j],
-                    g.conic[3 * (int64_t)j + 1], g.conic[3 * (int64_t)j + 2], g.opac[oi]};
+  bool reach;
+  L.t = tight_setup(u, v, g.conic[3 * (int64_t)j], g.conic[3 * (int64_t)j + 1],
+                    g.conic[3 * (int64_t)j + 2], g.opac[oi], reach);
+  L.x0 = q.x0;
+  L.y0 = q.y0;
+  L.w = max(q.x1 - q.x0, 1);
+  L.area = reach ? (q.x1 - q.x0) * (q.y1 - q.y0) : 0;
+  L.W = ig.W;
+  L.H = ig.H;
+  return L;
 }
 
-// tile (tx, ty) of a view of size W x H: pixel-centre box, then the test
-__device__ __forceinline__ bool tight_tile(const TightSplat &s, int tx, int ty, int tw, int th,
-                                           int W, int H) {
-  const int px0 = tx * tw, py0 = ty * th;
-  const int px1 = min(px0 + tw, W) - 1, py1 = min(py0 + th, H) - 1;
-  return tile_reachable(s.u, s.v, s.a, s.b, s.c, s.o, (float)px0, (float)px1, (float)py0,
-                        (float)py1);
+__device__ __forceinline__ float shfl_f(float x, int src) { return __shfl(x, src, 64); }
+__device__ __forceinline__ int shfl_i(int x, int src) { return __shfl(x, src, 64); }
+
+// The candidate c of the wave: its owner lane and whether the tile passes.
+struct TCand {
+  int owner, tx, ty;
+  bool pass;
+};
+
+__device__ __forceinline__ TCand tight_candidate(const TLane &L, int P, uint32_t c,
+                                                 uint32_t total, int tw, int th) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const int mid = lo + step;
+    if ((uint32_t)shfl_i(P, mid) <= c) lo = mid;
+  }
+  TightSplat t;
+  t.u = shfl_f(L.t.u, lo);
+  t.v = shfl_f(L.t.v, lo);
+  t.a = shfl_f(L.t.a, lo);
+  t.b = shfl_f(L.t.b, lo);
+  t.c = shfl_f(L.t.c, lo);
+  t.rbc = shfl_f(L.t.rbc, lo);
+  t.rba = shfl_f(L.t.rba, lo);
+  t.tau = shfl_f(L.t.tau, lo);
+  const int x0 = shfl_i(L.x0, lo), y0 = shfl_i(L.y0, lo), w = shfl_i(L.w, lo);
+  const int W = shfl_i(L.W, lo), H = shfl_i(L.H, lo), Po = shfl_i(P, lo);
+  TCand r;
+  r.owner = lo;
+  r.pass = false;
+  r.tx = r.ty = 0;
+  if (c < total) {
+    const int k = (int)c - Po;
+    const int row = (int)(((float)k + 0.5f) * (1.0f / (float)w));
+    const int col = k - row * w;
+    r.tx = x0 + col;
+    r.ty = y0 + row;
+    const int px0 = r.tx * tw, py0 = r.ty * th;
+    const int px1 = min(px0 + tw, W) - 1, py1 = min(py0 + th, H) - 1;
+    r.pass = tile_reachable(t, (float)px0, (float)px1, (float)py0, (float)py1);
+  }
+  return r;
+}
+
+// exclusive prefix of v over the wave; *total = the wave's sum
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+  const int lane = lane_id();
+  int inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  total = __shfl(inc, 63, 64);
+  return inc - v;
 }
 
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
@@ -333,15 +407,37 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
   if (r > 0) {
     const ItemGrid ig = item_grid(g, i);
     const Rect q = tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
-    if (g.conic == nullptr) {
-      c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
-    } else {
-      const TightSplat sp = tight_splat(g, uv, i);
-      for (int ty = q.y0; ty < q.y1; ++ty)
-        for (int tx = q.x0; tx < q.x1; ++tx) c += tight_tile(sp, tx, ty, g.tw, g.th, ig.W, ig.H);
-    }
+    c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
   }
   cnt[i] = c;
+}
+
+// Tight counts: per splat the number of rectangle tiles tile_reachable
+// accepts (flattened per wave; lane j adds the passes that fall in its own
+// candidate range of each 64-candidate chunk).
+__global__ __launch_bounds__(kThreads) void count_tight_kernel(const float *__restrict__ uv,
+                                                               const int32_t *__restrict__ radius,
+                                                               uint32_t n, Grid g,
+                                                               uint32_t *__restrict__ cnt) {
+  const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
+  if (wave_base >= n) return;  // uniform per wave
+  const int lane = lane_id();
+  const uint32_t i = wave_base + lane;
+  const TLane L = tight_lane(g, uv, radius, i, i < n);
+  int total;
+  const int P = wave_excl_scan(L.area, total);
+  uint32_t mine = 0;
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const TCand cd = tight_candidate(L, P, (uint32_t)(c0 + lane), (uint32_t)total, g.tw, g.th);
+    const uint64_t bal = __ballot(cd.pass);
+    const int lo = max(P, c0) - c0, hi = min(P + L.area, c0 + 64) - c0;
+    if (hi > lo) {
+      const uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+      const uint64_t mask = up & ~((1ull << lo) - 1ull);
+      mine += (uint32_t)__popcll(bal & mask);
+    }
+  }
+  if (i < n) cnt[i] = mine;
 }
 
 struct LoadFlag {
@@ -419,10 +515,11 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
   }
 }
 
-// Tight emission: as emit_kernel, but for each splat the wave tests the
-// tiles of its rectangle 64 at a time and writes the accepted ones, in
-// row-major order, compacted with a ballot (count_kernel counted exactly the
-// same tests, so every splat fills [offset, offset + count)).
+// Tight emission: the wave's 64 depth-sorted splats own the contiguous
+// output range starting at the first one's offset; their accepted
+// candidates, in order, are compacted into it with a ballot (count_tight
+// accepted exactly the same candidates, so every splat fills
+// [offset, offset + count)).
 __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
     const float *__restrict__ uv, const int32_t *__restrict__ radius,
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
@@ -434,67 +531,33 @@ __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
   const uint32_t s = wave_base + lane;
-  int id = 0, x0 = 0, y0 = 0, w = 1, area = 0, gxi = 0, toff = 0, Wv = 0, Hv = 0;
-  uint32_t off = 0;
-  TightSplat sp{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (s < nv) {
-    id = sorted_ids[s];
+  const bool valid = s < nv;
+  const int id = valid ? sorted_ids[s] : 0;
+  const TLane L = tight_lane(g, uv, radius, (uint32_t)id, valid);
+  int gxi = 0, toff = 0;
+  if (valid) {
     const ItemGrid ig = item_grid(g, (uint32_t)id);
     gxi = ig.gx;
     toff = ig.toff;
-    Wv = ig.W;
-    Hv = ig.H;
-    const Rect q = tile_rect(uv[2 * (int64_t)id], uv[2 * (int64_t)id + 1], radius[id], ig.gx,
-                             ig.gy, g.tw, g.th);
-    x0 = q.x0;
-    y0 = q.y0;
-    w = max(q.x1 - q.x0, 1);
-    area = (q.x1 - q.x0) * (q.y1 - q.y0);
-    off = offsets[s];
-    sp = tight_splat(g, uv, (uint32_t)id);
   }
-  const int nsplat = min(64, (int)(nv - wave_base));
-  for (int j = 0; j < nsplat; ++j) {
-    const int jid = __builtin_amdgcn_readlane(id, j);
-    const int jx0 = __builtin_amdgcn_readlane(x0, j);
-    const int jy0 = __builtin_amdgcn_readlane(y0, j);
-    const int jw = __builtin_amdgcn_readlane(w, j);
-    const int jarea = __builtin_amdgcn_readlane(area, j);
-    const int jgx = __builtin_amdgcn_readlane(gxi, j);
-    const int jtoff = __builtin_amdgcn_readlane(toff, j);
-    const int jW = __builtin_amdgcn_readlane(Wv, j);
-    const int jH = __builtin_amdgcn_readlane(Hv, j);
-    const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
-    TightSplat js;
-    js.u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.u), j));
-    js.v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.v), j));
-    js.a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.a), j));
-    js.b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.b), j));
-    js.c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.c), j));
-    js.o = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sp.o), j));
-    const float inv_w = 1.0f / (float)jw;
-    uint32_t run = 0;
-    for (int k0 = 0; k0 < jarea; k0 += 64) {
-      const int k = k0 + lane;
-      bool pass = false;
-      uint32_t key = 0;
-      if (k < jarea) {
-        const int row = (int)(((float)k + 0.5f) * inv_w);
-        const int col = k - row * jw;
-        pass = tight_tile(js, jx0 + col, jy0 + row, g.tw, g.th, jW, jH);
-        key = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
+  const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)offsets[wave_base]);
+  int total;
+  const int P = wave_excl_scan(L.area, total);
+  uint32_t run = 0;
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const TCand cd = tight_candidate(L, P, (uint32_t)(c0 + lane), (uint32_t)total, g.tw, g.th);
+    const int oid = shfl_i(id, cd.owner), ogx = shfl_i(gxi, cd.owner);
+    const int otoff = shfl_i(toff, cd.owner);
+    const uint64_t bal = __ballot(cd.pass);
+    if (cd.pass) {
+      // capacity guard as in emit_kernel
+      const uint32_t pos = base + run + (uint32_t)__popcll(bal & lt);
+      if (pos < cap_inst) {
+        tkeys[pos] = (uint32_t)(otoff + cd.ty * ogx + cd.tx);
+        tvals[pos] = oid;
       }
-      const uint64_t bal = __ballot(pass);
-      if (pass) {
-        // capacity guard as in emit_kernel
-        const uint32_t pos = joff + run + (uint32_t)__popcll(bal & lt);
-        if (pos < cap_inst) {
-          tkeys[pos] = key;
-          tvals[pos] = jid;
-        }
-      }
-      run += (uint32_t)__popcll(bal);
     }
+    run += (uint32_t)__popcll(bal);
   }
 }
 
@@ -639,7 +702,11 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     fill_kernel<<<1, kThreads, 0, s>>>(m_out, 1, 0);
     return gsr_launch_status("sort_count");
   }
-  count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
+  if (gr.conic == nullptr)
+    count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
+  else
+    count_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr,
+                                                                    g.cnt);
   // compact visible splats with their depth bits as keys -> Nv in scalars[0]
   run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
            g.sb.partials, g.scalars, nullptr, s);

@@ -63,37 +63,79 @@ __device__ __forceinline__ Rect tile_rect(float u, float v, int r, int gx, int g
   return q;
 }
 
-// Opacity-aware tile test.  Whether a splat centred at (u, v) with conic
-// (a, b, c) and opacity o can reach alpha >= kAlphaMin at some pixel centre of
-// the box [x0, x1] x [y0, y1] (inclusive pixel coordinates).  Alpha at pixel p
-// is o * exp(-q/2) with q = a dx^2 + 2 b dx dy + c dy^2, d = (u, v) - p, so the
-// test takes the minimum of the (convex) q over the continuous box -- zero if
-// the centre is inside, else the least of the four edge minima -- lowers it
-// by a margin (1e-3 + 1e-4 * S, S bounding |a| dx^2 + 2|b||dx dy| + |c| dy^2
-// on the box; far above the rounding of the blend's FP32 evaluation of q and
-// of exp) and compares alpha there with kAlphaMin.  Never false where a pixel
-// of the box reaches the threshold, so dropping the tiles it rejects leaves
-// every blended pixel unchanged.
-__device__ __forceinline__ float quad_edge_min(float a, float b, float c, float X, float y0,
-                                               float y1) {
-  // min over y in [y0, y1] of a X^2 + 2 b X y + c y^2 (c > 0)
-  const float y = fminf(fmaxf(-(b * X) / c, y0), y1);
-  return a * X * X + 2.0f * b * X * y + c * y * y;
+// ln(x) for x > 0 in normal range: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m-1)/(m+1), |s| < 0.172, by its odd series to s^9
+// (truncation < 4e-10).  frexp, one correctly rounded division, mul and fma:
+// bit-identical on the GPU and in a host C build.
+__device__ __forceinline__ float log_pos(float x) {
+  int e;
+  float m = frexpf(x, &e);
+  if (m < 0.70710678f) {
+    m = m * 2.0f;
+    e = e - 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float p = 1.0f / 9.0f;
+  p = fmaf(p, s2, 1.0f / 7.0f);
+  p = fmaf(p, s2, 1.0f / 5.0f);
+  p = fmaf(p, s2, 1.0f / 3.0f);
+  p = fmaf(p, s2, 1.0f);
+  return fmaf(2.0f * s, p, (float)e * 0.69314718f);
 }
 
-__device__ __forceinline__ bool tile_reachable(float u, float v, float a, float b, float c,
-                                               float o, float x0, float x1, float y0, float y1) {
-  if (!(a > 0.0f) || !(c > 0.0f)) return true;
-  const float dx0 = u - x1, dx1 = u - x0, dy0 = v - y1, dy1 = v - y0;
+// Opacity-aware tile test.  A splat centred at (u, v) with conic (a, b, c)
+// and opacity o reaches alpha >= kAlphaMin at pixel p only if
+// q = a dx^2 + 2 b dx dy + c dy^2 <= tau = 2 ln(o / kAlphaMin), d = (u,v) - p
+// (alpha = min(0.99, o exp(-q/2)); o < kAlphaMin reaches it nowhere).  The
+// test takes the minimum of the convex q over the tile's continuous
+// pixel-centre box -- zero with the centre inside, else the least of the four
+// edge minima (each a 1-D quadratic, minimiser -(b/c) X or -(b/a) Y clamped
+// to the edge) -- and keeps the tile when that minimum, lowered by the
+// margin 1e-3 + 1e-4 * S (S bounds |a| dx^2 + 2|b||dx dy| + |c| dy^2 on the
+// box: far above the rounding of the blend's FP32 q, of exp and of log_pos),
+// is <= tau.  Never false where a pixel of the box reaches the threshold, so
+// dropping the tiles it rejects leaves every blended pixel unchanged.
+struct TightSplat {
+  float u, v, a, b, c, rbc, rba, tau;
+};
+
+// reach == false: the splat reaches the threshold nowhere
+__device__ __forceinline__ TightSplat tight_setup(float u, float v, float a, float b, float c,
+                                                  float o, bool &reach) {
+  TightSplat t;
+  t.u = u;
+  t.v = v;
+  t.a = a;
+  t.b = b;
+  t.c = c;
+  reach = o >= kAlphaMin;
+  t.tau = reach ? 2.0f * log_pos(o / kAlphaMin) : 0.0f;
+  const bool pd = a > 0.0f && c > 0.0f;
+  t.rbc = pd ? b / c : 0.0f;
+  t.rba = pd ? b / a : 0.0f;
+  return t;
+}
+
+__device__ __forceinline__ bool tile_reachable(const TightSplat &t, float x0, float x1, float y0,
+                                               float y1) {
+  if (!(t.a > 0.0f) || !(t.c > 0.0f)) return true;
+  const float dx0 = t.u - x1, dx1 = t.u - x0, dy0 = t.v - y1, dy1 = t.v - y0;
   if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return true;
-  float m = quad_edge_min(a, b, c, dx0, dy0, dy1);
-  m = fminf(m, quad_edge_min(a, b, c, dx1, dy0, dy1));
-  m = fminf(m, quad_edge_min(c, b, a, dy0, dx0, dx1));
-  m = fminf(m, quad_edge_min(c, b, a, dy1, dx0, dx1));
+  float m;
+  {
+    const float ya = fminf(fmaxf(-(t.rbc * dx0), dy0), dy1);
+    const float yb = fminf(fmaxf(-(t.rbc * dx1), dy0), dy1);
+    const float xa = fminf(fmaxf(-(t.rba * dy0), dx0), dx1);
+    const float xb = fminf(fmaxf(-(t.rba * dy1), dx0), dx1);
+    m = t.a * dx0 * dx0 + 2.0f * t.b * dx0 * ya + t.c * ya * ya;
+    m = fminf(m, t.a * dx1 * dx1 + 2.0f * t.b * dx1 * yb + t.c * yb * yb);
+    m = fminf(m, t.c * dy0 * dy0 + 2.0f * t.b * dy0 * xa + t.a * xa * xa);
+    m = fminf(m, t.c * dy1 * dy1 + 2.0f * t.b * dy1 * xb + t.a * xb * xb);
+  }
   const float DX = fmaxf(fabsf(dx0), fabsf(dx1)), DY = fmaxf(fabsf(dy0), fabsf(dy1));
-  const float S = fabsf(a) * DX * DX + 2.0f * fabsf(b) * DX * DY + fabsf(c) * DY * DY;
-  const float t = fmaxf(m - (1e-3f + 1e-4f * S), 0.0f);
-  return o * exp_neg(-0.5f * t) >= kAlphaMin;
+  const float S = fabsf(t.a) * DX * DX + 2.0f * fabsf(t.b) * DX * DY + fabsf(t.c) * DY * DY;
+  return m - (1e-3f + 1e-4f * S) <= t.tau;
 }
 
 // ---------------------------------------------------------------------------

@@ -262,7 +262,8 @@ _default_ws = SortWorkspace()
 last_instance_count = 0  # M of the most recent sort_gaussian call (diagnostics)
 
 
-def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, workspace=None):
+def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, workspace=None,
+                  conic=None, opacity=None):
     """-> (gaussian_ids_sorted [M] int32, tile_range [T,2] int32).
 
     Order: (tile, float32 bits of depth, splat index).  One host read of the

@@ -46,7 +46,10 @@ def test_tight_sort_bitexact(device, n, w, h, seed):
     assert ids_r.shape[0] < it["ids"].shape[0]
 
 
-def grads_close(ga, gb, tol=1e-5):
+def grads_close(ga, gb, tol=1e-4):
+    # the two runs add the same per-pixel contributions into the per-splat
+    # records with float atomics in a different order (fewer instances, other
+    # tile lists): the north-star gradient bar, 1e-4 of max |g|, covers that
     for k in ga:
         e = (ga[k] - gb[k]).abs().max().item() / max(gb[k].abs().max().item(), 1e-30)
         assert e < tol, (k, e)

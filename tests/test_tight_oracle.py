@@ -109,3 +109,14 @@ def test_tight_sort_is_a_subsequence_and_blends_identically():
     out_t, T_t, _ = r32.alpha_blending(uv, conic, op, feat, ids_t, rng_t, 0.0, W, H)
     assert np.array_equal(out_e, out_t)
     assert np.array_equal(T_e, T_t)
+
+
+def test_log_pos_accuracy():
+    xs = np.concatenate([np.exp(np.linspace(-20, 20, 4001)), [1.0, 2.0, 0.5, 255.0, 1e-3]])
+    worst = 0.0
+    for x in xs.astype(np.float32):
+        got = r32.log_pos(float(x))
+        ref = math.log(float(x))
+        worst = max(worst, abs(got - ref) / max(abs(ref), 1.0))
+    assert worst < 3e-7, worst
+    assert r32.log_pos(1.0) == 0.0
