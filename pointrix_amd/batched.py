@@ -72,6 +72,30 @@ def view_floats(cameras) -> torch.Tensor:
     return torch.stack(rows)
 
 
+class _Unpack(torch.autograd.Function):
+    """Packed [C, total_pixels] -> the V views' [C, H, W] (views, no copy).
+    The backward writes the views' gradients into one packed buffer -- one
+    copy per view -- instead of autograd's full-size zero tensor per sliced
+    view."""
+
+    @staticmethod
+    def forward(ctx, packed, layout):
+        ctx.layout, ctx.shape = layout, packed.shape
+        return tuple(layout.unpack(packed, v) for v in range(layout.V))
+
+    @staticmethod
+    def backward(ctx, *grads):
+        lay = ctx.layout
+        g = torch.empty(ctx.shape, device=next(x for x in grads if x is not None).device)
+        for v, gv in enumerate(grads):
+            dst = lay.unpack(g, v)
+            if gv is None:
+                dst.zero_()
+            else:
+                dst.copy_(gv)
+        return g, None
+
+
 class InstanceBudgetExceeded(RuntimeError):
     """The batch needs more (splat, tile) instances than the renderer's
     ``max_instances`` allows; raised after counting, before allocating."""
@@ -200,12 +224,13 @@ class BatchedRender:
     """
 
     def __init__(self, render_depth=False, render_normal=False, white_bg=False, sh_degree=3,
-                 tile=(16, 16), batch_depth=256):
+                 tile=(16, 16), batch_depth=256, render_opacity=False):
         if tuple(tile) not in TILE_SHAPES or batch_depth not in BATCH_DEPTHS:
             raise ValueError(f"tile must be one of {TILE_SHAPES}, batch_depth one of "
                              f"{BATCH_DEPTHS}")
         self.render_depth = render_depth
         self.render_normal = render_normal
+        self.render_opacity = render_opacity   # per view "opacity" = 1 - T_final
         self.bg = 1.0 if white_bg else 0.0
         self.sh_degree = sh_degree
         self.tile = tuple(tile)
@@ -249,9 +274,11 @@ class BatchedRender:
         autograd (torch.no_grad) no ``uv_points`` leaf is allocated."""
         out, T, radius, ndc, lay = self.render_packed(cameras, position, opacity, scaling,
                                                       rotation, shs, layout, vi, half, vf)
+        imgs = _Unpack.apply(out, lay)
+        Ts = _Unpack.apply(T[None], lay) if self.render_opacity else None
         views = []
         for v in range(lay.V):
-            img = lay.unpack(out, v)
+            img = imgs[v]
             d, start = {}, 0
             for name, c in self.channel_names():
                 d[name] = img[start:start + c]
@@ -259,7 +286,8 @@ class BatchedRender:
             if "normal" in d:
                 nrm = d["normal"]
                 d["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
-            d["opacity"] = 1.0 - lay.unpack(T[None], v)
+            if Ts is not None:
+                d["opacity"] = 1.0 - Ts[v]
             views.append(d)
         rad = radius.view(lay.V, -1)
         return {"views": views, "uv_points": ndc, "visibility": (rad > 0).any(0),

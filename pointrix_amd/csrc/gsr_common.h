@@ -149,27 +149,59 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// Cross-lane exchange for the butterfly below, without the LDS crossbar
+// where the ISA allows: lanes L and L^M pair up.  M = 32 and 16 use gfx950's
+// v_permlane32_swap / v_permlane16_swap (swap the upper half-wave / odd rows
+// of the first operand with the lower half-wave / even rows of the second),
+// M = 8, 2, 1 use DPP (row rotate by 8, quad permutes), M = 4 a ds_swizzle
+// xor pattern.
+//
+// xpair<M>(lo, hi): a lane with bit M clear returns lo + (partner's lo); a
+// lane with bit M set returns (partner's hi) + hi.
+template <int M>
+__device__ __forceinline__ float xpair(float lo, float hi, int lane);
+
+template <int M>
+__device__ __forceinline__ float xor_get(float v) {
+  const int x = __float_as_int(v);
+  if constexpr (M == 1) return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+  else if constexpr (M == 2) return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+  else if constexpr (M == 8) return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+  else if constexpr (M == 4) return __int_as_float(__builtin_amdgcn_ds_swizzle(x, 0x101F));
+  else return __shfl_xor(v, M, 64);
+}
+
+template <int M>
+__device__ __forceinline__ float xpair(float lo, float hi, int lane) {
+  if constexpr (M == 32 || M == 16) {
+    const unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+    const auto r = M == 32 ? __builtin_amdgcn_permlane32_swap(a, b, false, false)
+                           : __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const bool up = (lane & M) != 0;
+    const float keep = up ? hi : lo, send = up ? lo : hi;
+    const float got = xor_get<M>(send);
+    return up ? got + keep : keep + got;
+  }
+}
+
 // Butterfly transpose reduction of N values per lane across the wave64.
 // Each xor stage M exchanges half of the live values with the partner lane
 // and adds, halving the live count; after log2(N) stages every lane holds one
 // fully or partially reduced value, and the remaining stages are plain
-// xor-adds.  Costs ~2N shuffles instead of 6N for N independent reductions.
+// xor-adds.  Costs ~2N exchanges instead of 6N for N independent reductions.
 // On return v[0] of lane L is the wave sum of value bfly_index<N>(L).
 template <int N, int M>
 struct Bfly {
   __device__ static __forceinline__ void run(float *v, int lane) {
     if constexpr (N > 1) {
       constexpr int H = N / 2;
-      const bool hi = (lane & M) != 0;
 #pragma unroll
-      for (int i = 0; i < H; ++i) {
-        const float keep = hi ? v[i + H] : v[i];
-        const float send = hi ? v[i] : v[i + H];
-        v[i] = keep + __shfl_xor(send, M, 64);
-      }
+      for (int i = 0; i < H; ++i) v[i] = xpair<M>(v[i], v[i + H], lane);
       Bfly<H, M / 2>::run(v, lane);
     } else if constexpr (M >= 1) {
-      v[0] += __shfl_xor(v[0], M, 64);
+      v[0] = xpair<M>(v[0], v[0], lane);
       Bfly<1, M / 2>::run(v, lane);
     }
   }

@@ -54,7 +54,8 @@ def test_batched_equals_per_view(device, bd):
     scene, cams = setup_case(device)
     # batched
     cb = cam_dicts(cams, grad=True)
-    r = BatchedRender(render_depth=True, render_normal=True, sh_degree=3, batch_depth=bd)
+    r = BatchedRender(render_depth=True, render_normal=True, sh_degree=3, batch_depth=bd,
+                      render_opacity=True)
     res = r.render(cb, **scene.render_inputs(split_sh=True))
     imgs_b = [{k: v for k, v in d.items()} for d in res["views"]]
     w = weights(imgs_b, 3)
