@@ -156,6 +156,9 @@ int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *con
  * 0, 0 (tiles of tile_w x tile_h; views' tiles and pixels numbered
  * consecutively).  Per-view outputs are indexed by j = view * N + splat;
  * images are packed [C, total_pixels].  V <= 64. */
+/* flags: 1 depth channel, 2 normal channel, 4 "sparse": culled (view, splat)
+ * pairs get radius = tiles = 0 and nothing else written (no later stage,
+ * backward included, reads anything else of them). */
 int gsr_mv_preprocess_fwd(const float *xyz, const float *scale, const float *rot,
                           const float *shs, const float *shs_rest, int64_t n, int K, int degree,
                           int V, const float *view_f, const int32_t *view_i, int tile_w,

@@ -140,8 +140,13 @@ class _BatchedFn(torch.autograd.Function):
         tiles = torch.empty(nv, dtype=torch.int32, device=dev)
         feat = torch.empty(nv, C, device=dev)
         with ops._span("mv_preprocess_fwd", dev):
+            # unless the intermediates are kept for inspection, culled (view,
+            # splat) pairs get only radius = tiles = 0 (the backward and every
+            # later stage read nothing else of them)
+            sparse = 4 if meta.get("keep") is None else 0
             call("gsr_mv_preprocess_fwd", ptr(x), ptr(s), ptr(q), ptr(dc), ptr(rest), n, K,
-                 int(degree), V, ptr(vfc), ptr(vi), lay.tile_w, lay.tile_h, 0.2, 1.3, int(flags),
+                 int(degree), V, ptr(vfc), ptr(vi), lay.tile_w, lay.tile_h, 0.2, 1.3,
+                 int(flags) | sparse,
                  ptr(uv), ptr(depth), ptr(conic), ptr(radius), ptr(tiles), ptr(feat), st)
         m_dev = meta["m_dev"]
         g = ws.sort.gauss_buf(nv, dev)

@@ -117,29 +117,48 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t *partials,
   }
 }
 
+// Downsweep: thread t owns items [16t, 16t + 16) of the block for the serial
+// part of the scan, but items are loaded and stored in coalesced order
+// (item r * 256 + t by thread t) through an LDS transpose (row padded to 17
+// words: conflict-free both ways).
 template <class Load, class Store>
 __global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, Store st,
                                                              const uint32_t *n_ptr, uint32_t cap,
                                                              const uint32_t *partials) {
+  constexpr int kRow = kItems + 1;
+  __shared__ uint32_t s_v[kThreads * kRow];
+  __shared__ uint32_t s_e[kThreads * kRow];
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
   if (base >= n) return;
-  const uint32_t first = base + threadIdx.x * kItems;
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t idx = r * kThreads + threadIdx.x;
+    const uint32_t i = base + idx;
+    s_v[(idx / kItems) * kRow + idx % kItems] = i < n ? ld(i) : 0u;
+  }
+  __syncthreads();
   uint32_t v[kItems];
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const uint32_t i = first + k;
-    v[k] = i < n ? ld(i) : 0u;
+    v[k] = s_v[threadIdx.x * kRow + k];
     s += v[k];
   }
   uint32_t total;
   uint32_t run = partials[blockIdx.x] + block_exclusive_scan(s, total);
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const uint32_t i = first + k;
-    if (i < n) st(i, run, v[k]);
+    s_e[threadIdx.x * kRow + k] = run;
     run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t idx = r * kThreads + threadIdx.x;
+    const uint32_t i = base + idx;
+    const uint32_t slot = (idx / kItems) * kRow + idx % kItems;
+    if (i < n) st(i, s_e[slot], s_v[slot]);
   }
 }
 
@@ -311,6 +330,7 @@ __device__ __forceinline__ ItemGrid item_grid(const Grid &g, uint32_t j) {
 struct TLane {
   TightSplat t;
   int x0, y0, w, area, W, H;
+  float inv_w;
 };
 
 __device__ __forceinline__ TLane tight_lane(const Grid &g, const float *uv,
@@ -319,6 +339,7 @@ __device__ __forceinline__ TLane tight_lane(const Grid &g, const float *uv,
   L.t = TightSplat{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   L.x0 = L.y0 = L.area = L.W = L.H = 0;
   L.w = 1;
+  L.inv_w = 1.0f;
   if (!valid) return L;
   const int r = radius[j];
   if (r <= 0) return L;
@@ -332,6 +353,7 @@ __device__ __forceinline__ TLane tight_lane(const Grid &g, const float *uv,
   L.x0 = q.x0;
   L.y0 = q.y0;
   L.w = max(q.x1 - q.x0, 1);
+  L.inv_w = 1.0f / (float)L.w;
   L.area = reach ? (q.x1 - q.x0) * (q.y1 - q.y0) : 0;
   L.W = ig.W;
   L.H = ig.H;
@@ -341,46 +363,43 @@ __device__ __forceinline__ TLane tight_lane(const Grid &g, const float *uv,
 __device__ __forceinline__ float shfl_f(float x, int src) { return __shfl(x, src, 64); }
 __device__ __forceinline__ int shfl_i(int x, int src) { return __shfl(x, src, 64); }
 
-// The candidate c of the wave: its owner lane and whether the tile passes.
-struct TCand {
-  int owner, tx, ty;
-  bool pass;
-};
-
-__device__ __forceinline__ TCand tight_candidate(const TLane &L, int P, uint32_t c,
-                                                 uint32_t total, int tw, int th) {
+// Owner lane of candidate c: the last lane with P <= c (P nondecreasing).
+__device__ __forceinline__ int tight_owner(int P, uint32_t c) {
   int lo = 0;
 #pragma unroll
   for (int step = 32; step >= 1; step >>= 1) {
     const int mid = lo + step;
     if ((uint32_t)shfl_i(P, mid) <= c) lo = mid;
   }
+  return lo;
+}
+
+// Tile of candidate k of a rectangle of width w: row = trunc((k + 0.5) / w)
+// in float (exact for every k of a tile grid), column = the rest.
+__device__ __forceinline__ void cand_tile(int k, int w, float inv_w, int x0, int y0, int &tx,
+                                          int &ty) {
+  const int row = (int)(((float)k + 0.5f) * inv_w);
+  tx = x0 + (k - row * w);
+  ty = y0 + row;
+}
+
+// The tile test of candidate (owner lane o, tile (tx, ty)); every lane takes
+// part in the shuffles.
+__device__ __forceinline__ bool owner_test(const TLane &L, int o, int tx, int ty, int tw,
+                                           int th) {
   TightSplat t;
-  t.u = shfl_f(L.t.u, lo);
-  t.v = shfl_f(L.t.v, lo);
-  t.a = shfl_f(L.t.a, lo);
-  t.b = shfl_f(L.t.b, lo);
-  t.c = shfl_f(L.t.c, lo);
-  t.rbc = shfl_f(L.t.rbc, lo);
-  t.rba = shfl_f(L.t.rba, lo);
-  t.tau = shfl_f(L.t.tau, lo);
-  const int x0 = shfl_i(L.x0, lo), y0 = shfl_i(L.y0, lo), w = shfl_i(L.w, lo);
-  const int W = shfl_i(L.W, lo), H = shfl_i(L.H, lo), Po = shfl_i(P, lo);
-  TCand r;
-  r.owner = lo;
-  r.pass = false;
-  r.tx = r.ty = 0;
-  if (c < total) {
-    const int k = (int)c - Po;
-    const int row = (int)(((float)k + 0.5f) * (1.0f / (float)w));
-    const int col = k - row * w;
-    r.tx = x0 + col;
-    r.ty = y0 + row;
-    const int px0 = r.tx * tw, py0 = r.ty * th;
-    const int px1 = min(px0 + tw, W) - 1, py1 = min(py0 + th, H) - 1;
-    r.pass = tile_reachable(t, (float)px0, (float)px1, (float)py0, (float)py1);
-  }
-  return r;
+  t.u = shfl_f(L.t.u, o);
+  t.v = shfl_f(L.t.v, o);
+  t.a = shfl_f(L.t.a, o);
+  t.b = shfl_f(L.t.b, o);
+  t.c = shfl_f(L.t.c, o);
+  t.rbc = shfl_f(L.t.rbc, o);
+  t.rba = shfl_f(L.t.rba, o);
+  t.tau = shfl_f(L.t.tau, o);
+  const int W = shfl_i(L.W, o), H = shfl_i(L.H, o);
+  const int px0 = tx * tw, py0 = ty * th;
+  const int px1 = min(px0 + tw, W) - 1, py1 = min(py0 + th, H) - 1;
+  return tile_reachable(t, (float)px0, (float)px1, (float)py0, (float)py1);
 }
 
 // exclusive prefix of v over the wave; *total = the wave's sum
@@ -418,7 +437,8 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
 __global__ __launch_bounds__(kThreads) void count_tight_kernel(const float *__restrict__ uv,
                                                                const int32_t *__restrict__ radius,
                                                                uint32_t n, Grid g,
-                                                               uint32_t *__restrict__ cnt) {
+                                                               uint32_t *__restrict__ cnt,
+                                                               uint64_t *__restrict__ masks) {
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
   if (wave_base >= n) return;  // uniform per wave
   const int lane = lane_id();
@@ -427,17 +447,28 @@ __global__ __launch_bounds__(kThreads) void count_tight_kernel(const float *__re
   int total;
   const int P = wave_excl_scan(L.area, total);
   uint32_t mine = 0;
+  uint64_t mbits = 0;
   for (int c0 = 0; c0 < total; c0 += 64) {
-    const TCand cd = tight_candidate(L, P, (uint32_t)(c0 + lane), (uint32_t)total, g.tw, g.th);
-    const uint64_t bal = __ballot(cd.pass);
+    const uint32_t c = (uint32_t)(c0 + lane);
+    const int o = tight_owner(P, c);
+    const int Po = shfl_i(P, o), w = shfl_i(L.w, o), x0 = shfl_i(L.x0, o), y0 = shfl_i(L.y0, o);
+    const float inv_w = shfl_f(L.inv_w, o);
+    int tx, ty;
+    cand_tile((int)c - Po, w, inv_w, x0, y0, tx, ty);
+    const bool test = owner_test(L, o, tx, ty, g.tw, g.th);
+    const uint64_t bal = __ballot(c < (uint32_t)total && test);
     const int lo = max(P, c0) - c0, hi = min(P + L.area, c0 + 64) - c0;
     if (hi > lo) {
       const uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
-      const uint64_t mask = up & ~((1ull << lo) - 1ull);
-      mine += (uint32_t)__popcll(bal & mask);
+      const uint64_t piece = (bal & up) >> lo;
+      mine += (uint32_t)__popcll(piece);
+      mbits |= piece << (max(P, c0) - P);  // only kept for areas <= 64
     }
   }
-  if (i < n) cnt[i] = mine;
+  if (i < n) {
+    cnt[i] = mine;
+    if (L.area > 0 && L.area <= 64) masks[i] = mbits;
+  }
 }
 
 struct LoadFlag {
@@ -523,8 +554,8 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
 __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
     const float *__restrict__ uv, const int32_t *__restrict__ radius,
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
-    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t cap_inst,
-    uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
+    const uint64_t *__restrict__ masks, const uint32_t *nv_ptr, uint32_t cap, Grid g,
+    uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
   const uint32_t nv = count_of(nv_ptr, cap);
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
   if (wave_base >= nv) return;  // uniform per wave
@@ -535,25 +566,43 @@ __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
   const int id = valid ? sorted_ids[s] : 0;
   const TLane L = tight_lane(g, uv, radius, (uint32_t)id, valid);
   int gxi = 0, toff = 0;
+  uint64_t mine = 0;
   if (valid) {
     const ItemGrid ig = item_grid(g, (uint32_t)id);
     gxi = ig.gx;
     toff = ig.toff;
+    if (L.area > 0 && L.area <= 64) mine = masks[id];
   }
+  const int mlo = (int)(uint32_t)mine, mhi = (int)(uint32_t)(mine >> 32);
   const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)offsets[wave_base]);
   int total;
   const int P = wave_excl_scan(L.area, total);
   uint32_t run = 0;
   for (int c0 = 0; c0 < total; c0 += 64) {
-    const TCand cd = tight_candidate(L, P, (uint32_t)(c0 + lane), (uint32_t)total, g.tw, g.th);
-    const int oid = shfl_i(id, cd.owner), ogx = shfl_i(gxi, cd.owner);
-    const int otoff = shfl_i(toff, cd.owner);
-    const uint64_t bal = __ballot(cd.pass);
-    if (cd.pass) {
+    const uint32_t c = (uint32_t)(c0 + lane);
+    const bool in = c < (uint32_t)total;
+    const int o = tight_owner(P, c);
+    const int Po = shfl_i(P, o), w = shfl_i(L.w, o), x0 = shfl_i(L.x0, o), y0 = shfl_i(L.y0, o);
+    const int area_o = shfl_i(L.area, o);
+    const float inv_w = shfl_f(L.inv_w, o);
+    const uint64_t mo = (uint64_t)(uint32_t)shfl_i(mlo, o) |
+                        ((uint64_t)(uint32_t)shfl_i(mhi, o) << 32);
+    int tx, ty;
+    const int k = (int)c - Po;
+    cand_tile(k, w, inv_w, x0, y0, tx, ty);
+    // areas <= 64: the count kernel's recorded decisions; larger: the test
+    bool pass = in && area_o <= 64 && ((mo >> (k & 63)) & 1ull);
+    if (__ballot(in && area_o > 64) != 0ull) {  // uniform
+      const bool test = owner_test(L, o, tx, ty, g.tw, g.th);
+      pass = pass || (in && area_o > 64 && test);
+    }
+    const int oid = shfl_i(id, o), ogx = shfl_i(gxi, o), otoff = shfl_i(toff, o);
+    const uint64_t bal = __ballot(pass);
+    if (pass) {
       // capacity guard as in emit_kernel
       const uint32_t pos = base + run + (uint32_t)__popcll(bal & lt);
       if (pos < cap_inst) {
-        tkeys[pos] = (uint32_t)(otoff + cd.ty * ogx + cd.tx);
+        tkeys[pos] = (uint32_t)(otoff + ty * ogx + tx);
         tvals[pos] = oid;
       }
     }
@@ -663,12 +712,13 @@ int tile_bits(int T) {
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
   uint32_t *cnt, *offs, *scalars;  // scalars[0] = Nv, scalars[1] = M
+  uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
   SortBufs sb;
 };
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
-  return 2 * align256(c * 4) + align256(64) + sortbufs_bytes(c);
+  return 2 * align256(c * 4) + align256(64) + align256(c * 8) + sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -677,6 +727,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 4, (void **)&g.cnt);
   p = carve(p, c * 4, (void **)&g.offs);
   p = carve(p, 64, (void **)&g.scalars);
+  p = carve(p, c * 8, (void **)&g.masks);
   carve_sortbufs(p, c, g.sb);
 }
 
@@ -706,7 +757,7 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
   else
     count_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr,
-                                                                    g.cnt);
+                                                                    g.cnt, g.masks);
   // compact visible splats with their depth bits as keys -> Nv in scalars[0]
   run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
            g.sb.partials, g.scalars, nullptr, s);
@@ -741,7 +792,8 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
                                                              (uint32_t)m, ib.keyA, v_first);
   else
     emit_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(
-        uv, radius, g.sb.valA, g.offs, g.scalars, (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
+        uv, radius, g.sb.valA, g.offs, g.masks, g.scalars, (uint32_t)n, gr, (uint32_t)m, ib.keyA,
+        v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
   run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);

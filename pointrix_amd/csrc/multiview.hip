@@ -30,6 +30,10 @@ constexpr int kVF = 20;
 constexpr int kVI = 8;
 constexpr int kFlagDepth = 1;
 constexpr int kFlagNormal = 2;
+// forward only, intermediates not inspected: for culled (view, splat) pairs
+// write only radius = tiles = 0 (nothing downstream reads their uv, depth,
+// conic or features), saving most of the per-pair output traffic
+constexpr int kFlagSparse = 4;
 
 __device__ __forceinline__ CamV cam_from(const float *p) {
   CamV c;
@@ -148,14 +152,15 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_fwd_kernel(
         }
       }
     }
+    radius[j] = r;
+    tiles[j] = t;
+    if (r == 0 && (flags & kFlagSparse)) continue;
     uv[2 * j] = u;
     uv[2 * j + 1] = vv;
     depth[j] = d;
     conic[3 * j] = con[0];
     conic[3 * j + 1] = con[1];
     conic[3 * j + 2] = con[2];
-    radius[j] = r;
-    tiles[j] = t;
     for (int c = 0; c < C; ++c) feat[j * C + c] = f[c];
   }
 }
@@ -453,7 +458,7 @@ int gsr_mv_preprocess_fwd(const float *xyz, const float *scale, const float *rot
                           int tile_h, float nearest, float extent, int flags, float *uv,
                           float *depth, float *conic, int32_t *radius, int32_t *tiles,
                           float *feat, void *stream) {
-  GSR_REQUIRE(n >= 0 && V >= 1 && V <= kMaxViews && (flags & ~3) == 0 && tile_w > 0 &&
+  GSR_REQUIRE(n >= 0 && V >= 1 && V <= kMaxViews && (flags & ~7) == 0 && tile_w > 0 &&
                   tile_h > 0,
               "mv_preprocess_fwd: bad arguments (V=%d)", V);
   if (n == 0) return GSR_OK;

@@ -107,11 +107,13 @@ class BatchServer:
     """Continuous batching (see the module docstring).  ``inputs``: the
     resident scene as render_iter takes it (activated tensors on the device;
     ``shs`` may be the (dc, rest) pair).  ``slots`` batches may be in flight
-    at once, each on its own stream.  ``tight_tiles``: opacity-aware binning
-    (the same images with fewer duplicated keys)."""
+    at once, each on its own stream.  Batches of at least ``tight_min_views``
+    views use opacity-aware binning (the same images with fewer duplicated
+    keys; it pays off on large batches, its extra tests cost more than they
+    save on a few views)."""
 
     def __init__(self, inputs: dict, sh_degree=3, key_budget_bytes=4 << 30, max_views=MAX_VIEWS,
-                 slots=2, tile=(16, 16), batch_depth=256, white_bg=False, tight_tiles=True):
+                 slots=2, tile=(16, 16), batch_depth=256, white_bg=False, tight_min_views=4):
         self.inputs = inputs
         self.dev = inputs["position"].device
         self.slots = []
@@ -119,11 +121,11 @@ class BatchServer:
             rs = {depth: BatchedRender(render_depth=depth, white_bg=white_bg,
                                        sh_degree=sh_degree, tile=tile, batch_depth=batch_depth)
                   for depth in (False, True)}
-            for r in rs.values():
-                r.tight_tiles = tight_tiles
+
             self.slots.append({"stream": torch.cuda.Stream(device=self.dev), "renderers": rs,
                                "busy": None})
         self.max_views = min(max_views, MAX_VIEWS)
+        self.tight_min_views = tight_min_views
         self.max_instances = max(1, key_budget_bytes // instance_bytes())
         self.per_pixel = None      # running instances per pixel (estimate)
         self.splits = 0
@@ -166,6 +168,7 @@ class BatchServer:
         while True:
             need_depth = any(r.depth for r in batch)
             rd = sl["renderers"][need_depth]
+            rd.tight_tiles = len(batch) >= self.tight_min_views
             room = self.max_instances - self.in_flight_instances()
             rd.max_instances = max(room, 1) if len(batch) > 1 else None
             lay = ViewLayout([r.width for r in batch], [r.height for r in batch], rd.tile[0],

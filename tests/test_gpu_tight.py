@@ -46,19 +46,26 @@ def test_tight_sort_bitexact(device, n, w, h, seed):
     assert ids_r.shape[0] < it["ids"].shape[0]
 
 
-def grads_close(ga, gb, tol=1e-4):
-    # the two runs add the same per-pixel contributions into the per-splat
-    # records with float atomics in a different order (fewer instances, other
-    # tile lists): the north-star gradient bar, 1e-4 of max |g|, covers that
-    for k in ga:
-        e = (ga[k] - gb[k]).abs().max().item() / max(gb[k].abs().max().item(), 1e-30)
-        assert e < tol, (k, e)
+def rel(ga, gb):
+    return {k: (ga[k] - gb[k]).abs().max().item() / max(gb[k].abs().max().item(), 1e-30)
+            for k in gb}
+
+
+def grads_close(g_tight, g_exact, g_exact2):
+    """The two binnings add the same per-(wave, splat) partial sums into the
+    per-splat records with float atomics, in another order; two exact runs
+    differ the same way.  Bound: the north-star 1e-4 of max |g|, or four
+    times the measured exact-vs-exact spread where summation cancellation
+    makes that spread larger."""
+    e, spread = rel(g_tight, g_exact), rel(g_exact2, g_exact)
+    for k in e:
+        assert e[k] < max(1e-4, 4.0 * spread[k]), (k, e[k], spread[k])
 
 
 def test_tight_single_view_same_images(device):
     scene, cam = scene_cam(device, 60000, 3, 640, 360)
     out = {}
-    for tight in (False, True):
+    for key, tight in (("exact", False), ("tight", True), ("exact2", False)):
         r = MsplatRender({"render_depth": True, "render_normal": True, "max_sh_degree": 3,
                           "tight_tiles": tight}, device="cuda")
         r.sh_degree = 3
@@ -69,13 +76,13 @@ def test_tight_single_view_same_images(device):
         g = torch.Generator().manual_seed(9)
         loss = sum((v * torch.randn(v.shape, generator=g).to(device)).sum() for v in f.values())
         loss.backward()
-        out[tight] = ({k: v.detach().clone() for k, v in f.items()},
-                      {k: p.grad.clone() for k, p in scene.named_params().items()},
-                      ops.last_instance_count)
-    for k in out[False][0]:
-        assert torch.equal(out[False][0][k], out[True][0][k]), k
-    grads_close(out[True][1], out[False][1])
-    assert out[True][2] < out[False][2]
+        out[key] = ({k: v.detach().clone() for k, v in f.items()},
+                    {k: p.grad.clone() for k, p in scene.named_params().items()},
+                    ops.last_instance_count)
+    for k in out["exact"][0]:
+        assert torch.equal(out["exact"][0][k], out["tight"][0][k]), k
+    grads_close(out["tight"][1], out["exact"][1], out["exact2"][1])
+    assert out["tight"][2] < out["exact"][2]
 
 
 @pytest.mark.parametrize("tile", [(16, 16), (8, 32), (32, 8)])
@@ -87,9 +94,9 @@ def test_tight_batched_same_images(device, tile):
         eye = S.random_sphere_eye(g, 2.5, 4.0)
         cams.append(S.make_camera(w, h, eye=eye, focal=1.1 * w).to(device).render_kwargs())
     res = {}
-    for tight in (False, True):
+    for tight in (False, True, "exact2"):
         r = BatchedRender(render_depth=True, sh_degree=3, tile=tile)
-        r.tight_tiles = tight
+        r.tight_tiles = tight is True
         for p in scene.params():
             p.grad = None
         out = r.render(cams, **scene.render_inputs(split_sh=True))
@@ -99,5 +106,5 @@ def test_tight_batched_same_images(device, tile):
                       {k: p.grad.clone() for k, p in scene.named_params().items()},
                       r.instances())
     assert torch.equal(res[False][0], res[True][0])
-    grads_close(res[True][1], res[False][1])
+    grads_close(res[True][1], res[False][1], res["exact2"][1])
     assert res[True][2] < res[False][2]
