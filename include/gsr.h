@@ -246,6 +246,26 @@ int gsr_l1_ssim_fwd(const float *x, const float *y, int planes, int W, int H, fl
 int gsr_l1_ssim_bwd(const float *x, const float *y, int planes, int W, int H, const float *g,
                     float *dx, void *stream);
 
+/* ---- fused Adam (torch.optim.Adam as pointrix configures it,
+ * pointrix/optimizer/__init__.py:61-63, optimizer.py:128-140; weight decay 0,
+ * no amsgrad) over every tensor of a step in one launch.  table[ntab] lives in
+ * device memory; entry k covers workgroups [first_chunk, first_chunk +
+ * ceil(n / gsr_adam_chunk())), nchunks is the total.  Per element:
+ *   m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+ *   p -= step_size * m / (sqrt(v) / bc2_sqrt + eps)
+ * with step_size = lr / (1 - b1^t), bc2_sqrt = sqrt(1 - b2^t); zero_grad != 0
+ * also writes g = 0. */
+typedef struct {
+  float *p, *g, *m, *v;
+  int64_t n;
+  int64_t first_chunk;
+  float step_size, bc2_sqrt;
+} GsrAdamTensor;
+int gsr_adam_chunk(void);
+int gsr_adam_entry_bytes(void); /* sizeof(GsrAdamTensor), for bindings */
+int gsr_adam_step(const GsrAdamTensor *table, int ntab, int64_t nchunks, double beta1,
+                  double beta2, double eps, int zero_grad, void *stream);
+
 /* ---- densification statistics (controller/gs.py:318-333, 259-284) -------
  * For every splat with visibility[i] != 0: g = (sum over the V views of
  * grad[v*view_stride + i*row_stride + {0,1}]) * (sx, sy) (the views summed in
@@ -256,6 +276,20 @@ int gsr_densify_stats(const float *grad, int64_t view_stride, int64_t row_stride
                       int64_t n, const uint8_t *visibility, const int32_t *radii, float sx,
                       float sy, float *grad_accum, float *acc_steps, float *max_radii,
                       void *stream);
+/* Densify / prune with optimizer-state remap (point_utils.py:51-108,
+ * controller/gs.py:182-247) as one row gather: for every entry of table
+ * (device memory; src [*, width], dst [rows, width] float32) dst row r = src
+ * row index[r], or zeros where index[r] < 0 (new rows' Adam moments);
+ * rows * width < 2^32 for every entry. */
+typedef struct {
+  const float *src;
+  float *dst;
+  int64_t width;
+} GsrRowCopy;
+int gsr_row_copy_bytes(void); /* sizeof(GsrRowCopy), for bindings */
+int gsr_gather_rows(const GsrRowCopy *table, int ntab, const int32_t *index, int64_t rows,
+                    void *stream);
+
 /* uv_points.grad [n,2] = the (du, dv) of the blend backward's records
  * (d_splat[i*ld + 0..1]) scaled by (sx, sy) = (W/2, H/2) (msplat.py:141-145). */
 int gsr_uv_grad_scale(const float *d_splat, int ld, int64_t n, float sx, float sy, float *out,

@@ -18,6 +18,7 @@ _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I = ctypes.c_int
 _F = ctypes.c_float
+_D = ctypes.c_double
 _SZ = ctypes.c_size_t
 
 # name -> (restype, argtypes); the single source of truth for the Python side
@@ -58,6 +59,11 @@ SIGNATURES = {
     "gsr_ssim_partials": (_I, [_I, _I, _I]),
     "gsr_l1_ssim_fwd": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
     "gsr_l1_ssim_bwd": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
+    "gsr_adam_chunk": (_I, []),
+    "gsr_adam_entry_bytes": (_I, []),
+    "gsr_adam_step": (_I, [_P, _I, _I64, _D, _D, _D, _I, _P]),
+    "gsr_row_copy_bytes": (_I, []),
+    "gsr_gather_rows": (_I, [_P, _I, _P, _I64, _P]),
     "gsr_densify_stats": (_I, [_P, _I64, _I64, _I, _I64, _P, _P, _F, _F, _P, _P, _P, _P]),
     "gsr_uv_grad_scale": (_I, [_P, _I, _I64, _F, _F, _P, _P]),
     "gsr_l1_loss_partials": (_I, [_I64]),

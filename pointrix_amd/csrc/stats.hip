@@ -72,3 +72,47 @@ int gsr_uv_grad_scale(const float *rec, int ld, int64_t n, float sx, float sy, f
 }
 
 }  // extern "C"
+
+// ---- row gather for densify / prune with optimizer-state remap --------------
+// dst row r of every table entry = src row index[r] (index < 0: zeros), for
+// all entries (parameters and their Adam moments) in one launch: the
+// compaction / extension pointrix does per tensor with boolean-mask indexing
+// and torch.cat (model/point_cloud/utils/point_utils.py:51-108,
+// controller/gs.py:182-247).
+namespace {
+
+__global__ __launch_bounds__(256) void gather_rows_kernel(const GsrRowCopy *__restrict__ tab,
+                                                          const int32_t *__restrict__ index,
+                                                          int64_t rows) {
+  const GsrRowCopy t = tab[blockIdx.y];
+  const int64_t total = rows * t.width;
+  const uint32_t w = (uint32_t)t.width;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    // rows * width < 2^32 (required of the caller): 32-bit division
+    const uint32_t e32 = (uint32_t)e;
+    const uint32_t r = e32 / w, c = e32 - r * w;
+    const int32_t s = index[r];
+    t.dst[e] = s >= 0 ? t.src[(int64_t)s * t.width + c] : 0.0f;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsr_gather_rows(const GsrRowCopy *table, int ntab, const int32_t *index, int64_t rows,
+                    void *stream) {
+  GSR_REQUIRE(ntab >= 0 && ntab <= 65535 && rows >= 0 && rows < (int64_t)INT32_MAX,
+              "gather_rows: bad sizes");
+  if (ntab == 0 || rows == 0) return GSR_OK;
+  const int64_t want = (rows * 16 + 255) / 256;
+  const unsigned gx = (unsigned)(want < 65536 ? want : 65536);
+  gather_rows_kernel<<<dim3(gx, (unsigned)ntab), 256, 0, (hipStream_t)stream>>>(table, index,
+                                                                              rows);
+  return gsr_launch_status("gather_rows");
+}
+
+int gsr_row_copy_bytes(void) { return (int)sizeof(GsrRowCopy); }
+
+}  // extern "C"
