@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -357,6 +358,96 @@ def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
             "_tune": ((tile, bd), tune)}
 
 
+def bench_train500k(steps, warmup, log):
+    """500k splats (SH 3), one 979x546 view, loss
+    0.8 L1 + 0.2 (1 - SSIM), densification statistics, Adam (pointrix's
+    per-attribute learning rates, eps 1e-15): the full training step.
+    fused = ops.l1_ssim_loss + DensifyStats + FusedAdam; torch = the same
+    step with the reference's torch formulations (model/loss.py SSIM with
+    conv2d, the controller's masked updates, torch.optim.Adam)."""
+    import torch.nn.functional as F
+
+    from pointrix_amd import scene as S
+    from pointrix_amd.densify import DensifyStats
+    from pointrix_amd.ops import l1_ssim_loss
+    from pointrix_amd.optim import FusedAdam
+    from pointrix_amd.renderer import MsplatRender
+
+    dev = torch.device("cuda:0")
+    scene_cpu, cams = S.train500k(0)
+    cam = cams[0].to(dev)
+    W, H = cam.width, cam.height
+    lrs = {"position": 1.6e-4, "features": 2.5e-3, "features_rest": 1.25e-4, "scaling": 5e-3,
+           "rotation": 1e-3, "opacity": 5e-2}
+    target = torch.rand(3, H, W, generator=torch.Generator().manual_seed(5)).to(dev)
+    # the unfused comparison: the reference's torch formulations, on the GPU
+    # (model/loss.py:27,67-116 SSIM by conv2d; controller/gs.py:318-333)
+    g1 = torch.tensor([math.exp(-(x - 5) ** 2 / 4.5) for x in range(11)])
+    g1 = g1 / g1.sum()
+    win = (g1[:, None] @ g1[None, :]).expand(3, 1, 11, 11).contiguous().to(dev)
+
+    def torch_loss(x, y):
+        x, y = x[None], y[None]
+        mu1, mu2 = F.conv2d(x, win, padding=5, groups=3), F.conv2d(y, win, padding=5, groups=3)
+        s11 = F.conv2d(x * x, win, padding=5, groups=3) - mu1 * mu1
+        s22 = F.conv2d(y * y, win, padding=5, groups=3) - mu2 * mu2
+        s12 = F.conv2d(x * y, win, padding=5, groups=3) - mu1 * mu2
+        smap = ((2 * mu1 * mu2 + 1e-4) * (2 * s12 + 9e-4)) / \
+            ((mu1 * mu1 + mu2 * mu2 + 1e-4) * (s11 + s22 + 9e-4))
+        return 0.8 * (x - y).abs().mean() + 0.2 * (1.0 - smap.mean())
+
+    def torch_stats(ga, st, mr, uv, vis, radii):
+        g = uv.grad.clone()
+        g[..., 0] *= W / 2.0
+        g[..., 1] *= H / 2.0
+        mr[vis] = torch.max(mr[vis], radii[vis].float())
+        ga[vis] += torch.norm(g[vis, :2], dim=-1, keepdim=True)
+        st[vis] += 1
+
+    out = {}
+    for mode in ("torch", "fused"):
+        scene = scene_cpu.to(dev, requires_grad=True)
+        named = scene.named_params()
+        groups = [{"params": [named[k]], "lr": lrs[k], "name": k} for k in named]
+        opt = FusedAdam(groups, eps=1e-15) if mode == "fused" else \
+            torch.optim.Adam(groups, eps=1e-15)
+        r = MsplatRender({"max_sh_degree": 3}, device="cuda")
+        r.sh_degree = 3
+        n = len(scene)
+        stats = DensifyStats(n, dev, W, H)
+        ga, st, mr = (torch.zeros(n, 1, device=dev), torch.zeros(n, 1, device=dev),
+                      torch.zeros(n, device=dev))
+
+        def step():
+            res = r.render_iter(**cam.render_kwargs(), **scene.render_inputs(split_sh=True))
+            rgb = res["rendered_features_split"]["rgb"]
+            if mode == "fused":
+                loss = l1_ssim_loss(rgb, target, 0.2)["loss"]
+            else:
+                loss = torch_loss(rgb, target)
+            loss.backward()
+            if mode == "fused":
+                stats.update(res["uv_points"], res["visibility"], res["radii"])
+            else:
+                torch_stats(ga, st, mr, res["uv_points"], res["visibility"], res["radii"])
+            opt.step()
+            opt.zero_grad(set_to_none=True)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        out[mode] = (time.perf_counter() - t0) * 1000 / steps
+        log(f"train500k {mode}: {out[mode]:.3f} ms/step")
+    return {"workload": "train500k: 500k splats SH3, 979x546, 0.8 L1 + 0.2 (1-SSIM), "
+                        "densification statistics, Adam",
+            "fused_ms_per_step": round(out["fused"], 3), "torch_ms_per_step": round(out["torch"], 3),
+            "speedup": round(out["torch"] / out["fused"], 3)}
+
+
 def bench_serve(log, n_requests=300, rates=(50.0, 200.0, 800.0)):
     """Serving: 2M resident splats (SH 3), Poisson request arrivals at each
     rate, resolutions {540p 40%, 720p 30%, 1080p 25%, 2160p 5%}, eye at
@@ -405,6 +496,8 @@ def main():
     ap.add_argument("--cpu-div", type=int, default=128)
     ap.add_argument("--batched", type=int, default=1, help="also run the 16-view batched case")
     ap.add_argument("--serve", type=int, default=1, help="also run the serving trace")
+    ap.add_argument("--train500k", type=int, default=1,
+                    help="also time the full config-2 training step (loss, stats, Adam)")
     ap.add_argument("--tight", type=int, default=1,
                     help="also time the step with opacity-aware binning")
     ap.add_argument("--graph", type=int, default=1,
@@ -524,6 +617,12 @@ def main():
         b3["autotune"] = {"chosen": b1["autotune"]["chosen"], "note": "choice of the 1M run"}
         batched = {"1M": b1, "3M": b3}
 
+    train500k = None
+    if args.train500k:
+        print("[bench] train500k full step ...", file=sys.stderr, flush=True)
+        train500k = bench_train500k(max(5, args.steps // 2), 3,
+                                    lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+
     serving = None
     if args.serve:
         print("[bench] serving ...", file=sys.stderr, flush=True)
@@ -551,6 +650,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "train500k": train500k,
     }
     print(json.dumps(line), flush=True)
 

@@ -154,6 +154,8 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     const float *__restrict__ d_feat, int ld_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
     float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_shs_rest,
     float *__restrict__ d_intr, float *__restrict__ d_extr, float *__restrict__ d_center) {
+  constexpr int kRest = (KS - 1) * 3;
+  __shared__ __align__(16) float s_rest[KS > 1 ? kThreads * kRest : 4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float gcam[16];
 #pragma unroll
@@ -248,9 +250,13 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       d_shs[3 * i] = dsh[0];
       d_shs[3 * i + 1] = dsh[1];
       d_shs[3 * i + 2] = dsh[2];
-      float *dr = d_shs_rest + i * ((KS - 1) * 3);
+      if constexpr (KS > 1) {
+        // this splat's row of the rest gradient, staged for the block's
+        // coalesced store below
 #pragma unroll
-      for (int j = 3; j < KS * 3; ++j) dr[j - 3] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+        for (int j = 3; j < KS * 3; ++j)
+          s_rest[threadIdx.x * kRest + (j - 3)] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+      }
     } else {
       float *ds = d_shs + i * (KS * 3);
       if constexpr ((KS * 3) % 4 == 0) {
@@ -267,6 +273,26 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       } else {
 #pragma unroll
         for (int j = 0; j < KS * 3; ++j) ds[j] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+      }
+    }
+  }
+  if constexpr (KS > 1) {
+    // The block's rows of d_shs_rest are one contiguous range: write it with
+    // consecutive lanes on consecutive 16-byte words (the rows are 12(K-1)
+    // bytes, so per-lane row stores would touch a new cache line per lane).
+    if (d_shs_rest != nullptr) {
+      __syncthreads();
+      const int64_t first = (int64_t)blockIdx.x * kThreads;
+      const int rows = (int)min((int64_t)kThreads, n - first);
+      const int total = rows * kRest;
+      float *dst = d_shs_rest + first * kRest;
+      if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
+        const float4 *s4 = reinterpret_cast<const float4 *>(s_rest);
+        for (int k = threadIdx.x; k < total / 4; k += kThreads) d4[k] = s4[k];
+        for (int k = (total / 4) * 4 + threadIdx.x; k < total; k += kThreads) dst[k] = s_rest[k];
+      } else {
+        for (int k = threadIdx.x; k < total; k += kThreads) dst[k] = s_rest[k];
       }
     }
   }

@@ -175,9 +175,11 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
     float *__restrict__ d_xyz, float *__restrict__ d_scale, float *__restrict__ d_rot,
     float *__restrict__ d_opacity, float *__restrict__ d_shs, float *__restrict__ d_shs_rest,
     float *__restrict__ d_vf) {
+  constexpr int kRest = (KS - 1) * 3;
   __shared__ float s_vf[kMaxViews * kVF];
   __shared__ int s_vi[kMaxViews * kVI];
   __shared__ float s_cam[kThreads / 64][32];
+  __shared__ __align__(16) float s_rest[KS > 1 ? kThreads * kRest : 4];
   for (int k = threadIdx.x; k < V * kVF; k += blockDim.x) s_vf[k] = vf[k];
   for (int k = threadIdx.x; k < V * kVI; k += blockDim.x) s_vi[k] = vi[k];
   __syncthreads();
@@ -289,25 +291,47 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
       __syncthreads();
     }
   }
-  if (!active) return;
-  float gs[3], gq2[4];
-  covariance3d_vjp(s, q, gS, gs, gq2);
-  d_xyz[3 * i] = gx.x;
-  d_xyz[3 * i + 1] = gx.y;
-  d_xyz[3 * i + 2] = gx.z;
-  d_opacity[i] = gop;
+  if (active) {
+    float gs[3], gq2[4];
+    covariance3d_vjp(s, q, gS, gs, gq2);
+    d_xyz[3 * i] = gx.x;
+    d_xyz[3 * i + 1] = gx.y;
+    d_xyz[3 * i + 2] = gx.z;
+    d_opacity[i] = gop;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
+    for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) d_rot[4 * i + k] = gq[k] + gq2[k];
+    for (int k = 0; k < 4; ++k) d_rot[4 * i + k] = gq[k] + gq2[k];
+  }
   if (d_shs_rest != nullptr) {
-    d_shs[3 * i] = dsh[0];
-    d_shs[3 * i + 1] = dsh[1];
-    d_shs[3 * i + 2] = dsh[2];
-    float *dr = d_shs_rest + i * ((KS - 1) * 3);
+    if (active) {
+      d_shs[3 * i] = dsh[0];
+      d_shs[3 * i + 1] = dsh[1];
+      d_shs[3 * i + 2] = dsh[2];
+    }
+    if constexpr (KS > 1) {
+      // stage the rows, then store the block's contiguous range of
+      // d_shs_rest with consecutive lanes on consecutive 16-byte words
+      if (active) {
 #pragma unroll
-    for (int j = 3; j < KS * 3; ++j) dr[j - 3] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
-  } else {
+        for (int j = 3; j < KS * 3; ++j)
+          s_rest[threadIdx.x * kRest + (j - 3)] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;
+      }
+      __syncthreads();
+      const int64_t first = (int64_t)blockIdx.x * kThreads;
+      const int rows = (int)min((int64_t)kThreads, n - first);
+      const int total = rows * kRest;
+      float *dst = d_shs_rest + first * kRest;
+      if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+        float4 *d4 = reinterpret_cast<float4 *>(dst);
+        const float4 *s4 = reinterpret_cast<const float4 *>(s_rest);
+        for (int k = threadIdx.x; k < total / 4; k += kThreads) d4[k] = s4[k];
+        for (int k = (total / 4) * 4 + threadIdx.x; k < total; k += kThreads) dst[k] = s_rest[k];
+      } else {
+        for (int k = threadIdx.x; k < total; k += kThreads) dst[k] = s_rest[k];
+      }
+    }
+  } else if (active) {
     float *ds = d_shs + i * (KS * 3);
 #pragma unroll
     for (int j = 0; j < KS * 3; ++j) ds[j] = (j < KE * 3) ? dsh[j % (KE * 3)] : 0.0f;

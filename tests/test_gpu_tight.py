@@ -18,9 +18,10 @@ from pointrix_amd.renderer import MsplatRender
 pytestmark = pytest.mark.gpu
 
 
-def scene_cam(device, n, seed, w, h):
+def scene_cam(device, n, seed, w, h, opacity_std=2.0):
     scene = S.random_scene(n, seed, sh_degree=3, means="normal", means_std=(1.5, 1.0, 1.5),
-                           log_scale=(-3.8, 0.8), opacity_std=2.0).to(device, requires_grad=True)
+                           log_scale=(-3.8, 0.8),
+                           opacity_std=opacity_std).to(device, requires_grad=True)
     g = torch.Generator().manual_seed(seed)
     cam = S.make_camera(w, h, eye=S.random_sphere_eye(g, 2.5, 4.0), focal=1.1 * w).to(device)
     return scene, cam
@@ -63,7 +64,7 @@ def grads_close(g_tight, g_exact, g_exact2):
 
 
 def test_tight_single_view_same_images(device):
-    scene, cam = scene_cam(device, 60000, 3, 640, 360)
+    scene, cam = scene_cam(device, 20000, 3, 640, 360, opacity_std=1.0)
     out = {}
     for key, tight in (("exact", False), ("tight", True), ("exact2", False)):
         r = MsplatRender({"render_depth": True, "render_normal": True, "max_sh_degree": 3,
