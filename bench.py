@@ -93,14 +93,16 @@ def l1(a, b):
     return l1_loss(a, b)
 
 
-def make_step(renderer, scene, cam, targets):
+def make_step(renderer, scene, cam, targets, raw=False):
     """-> (step, loss_fn): ``step(stats)`` = one eager fwd + loss + bwd with
     the gradients dropped afterwards; ``loss_fn()`` = fwd + loss + bwd leaving
-    the gradients in place (what a captured step replays)."""
+    the gradients in place (what a captured step replays).  ``raw``: the
+    renderer takes the raw parameters and applies the activations in its
+    preprocess kernel."""
     params = scene.params() + cam.params()
 
     def loss_fn(stats=None):
-        inp = scene.render_inputs(split_sh=True)
+        inp = scene.render_inputs(split_sh=True, raw=raw)
         res = renderer.render_iter(**cam.render_kwargs(), **inp)
         f = res["rendered_features_split"]
         loss = l1(f["rgb"], targets["rgb"])
@@ -239,6 +241,22 @@ def time_tight(renderer, step, steps):
     return {"ms_per_step": round(ms, 4), "instances": m}
 
 
+def time_raw(renderer, scene, cam, targets, steps):
+    """The step with the activations (sigmoid, exp, normalize) applied inside
+    the preprocess kernel instead of by torch before render_iter."""
+    step, _ = make_step(renderer, scene, cam, targets, raw=True)
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    print(f"[bench] raw-parameter step: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"ms_per_step": round(ms, 4)}
+
+
 def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
     """1M splats (SH 3), 16 cameras of uneven resolution,
     summed L1 over views, fwd+bwd.  Plain = one view per launch
@@ -362,7 +380,8 @@ def bench_train500k(steps, warmup, log):
     """500k splats (SH 3), one 979x546 view, loss
     0.8 L1 + 0.2 (1 - SSIM), densification statistics, Adam (pointrix's
     per-attribute learning rates, eps 1e-15): the full training step.
-    fused = ops.l1_ssim_loss + DensifyStats + FusedAdam; torch = the same
+    fused = ops.l1_ssim_loss + DensifyStats + FusedAdam; fused_raw = the same
+    with the activations applied inside the preprocess kernel; torch = the
     step with the reference's torch formulations (model/loss.py SSIM with
     conv2d, the controller's masked updates, torch.optim.Adam)."""
     import torch.nn.functional as F
@@ -405,11 +424,12 @@ def bench_train500k(steps, warmup, log):
         st[vis] += 1
 
     out = {}
-    for mode in ("torch", "fused"):
+    for mode in ("torch", "fused", "fused_raw"):
         scene = scene_cpu.to(dev, requires_grad=True)
         named = scene.named_params()
         groups = [{"params": [named[k]], "lr": lrs[k], "name": k} for k in named]
-        opt = FusedAdam(groups, eps=1e-15) if mode == "fused" else \
+        fused = mode != "torch"
+        opt = FusedAdam(groups, eps=1e-15) if fused else \
             torch.optim.Adam(groups, eps=1e-15)
         r = MsplatRender({"max_sh_degree": 3}, device="cuda")
         r.sh_degree = 3
@@ -419,14 +439,15 @@ def bench_train500k(steps, warmup, log):
                       torch.zeros(n, device=dev))
 
         def step():
-            res = r.render_iter(**cam.render_kwargs(), **scene.render_inputs(split_sh=True))
+            res = r.render_iter(**cam.render_kwargs(),
+                                **scene.render_inputs(split_sh=True, raw=mode == "fused_raw"))
             rgb = res["rendered_features_split"]["rgb"]
-            if mode == "fused":
+            if fused:
                 loss = l1_ssim_loss(rgb, target, 0.2)["loss"]
             else:
                 loss = torch_loss(rgb, target)
             loss.backward()
-            if mode == "fused":
+            if fused:
                 stats.update(res["uv_points"], res["visibility"], res["radii"])
             else:
                 torch_stats(ga, st, mr, res["uv_points"], res["visibility"], res["radii"])
@@ -444,8 +465,11 @@ def bench_train500k(steps, warmup, log):
         log(f"train500k {mode}: {out[mode]:.3f} ms/step")
     return {"workload": "train500k: 500k splats SH3, 979x546, 0.8 L1 + 0.2 (1-SSIM), "
                         "densification statistics, Adam",
-            "fused_ms_per_step": round(out["fused"], 3), "torch_ms_per_step": round(out["torch"], 3),
-            "speedup": round(out["torch"] / out["fused"], 3)}
+            "fused_ms_per_step": round(out["fused"], 3),
+            "fused_raw_ms_per_step": round(out["fused_raw"], 3),
+            "torch_ms_per_step": round(out["torch"], 3),
+            "speedup": round(out["torch"] / out["fused"], 3),
+            "speedup_raw": round(out["torch"] / out["fused_raw"], 3)}
 
 
 def bench_serve(log, n_requests=300, rates=(50.0, 200.0, 800.0)):
@@ -500,6 +524,8 @@ def main():
                     help="also time the full config-2 training step (loss, stats, Adam)")
     ap.add_argument("--tight", type=int, default=1,
                     help="also time the step with opacity-aware binning")
+    ap.add_argument("--raw", type=int, default=1,
+                    help="also time the step with activations fused into preprocess")
     ap.add_argument("--graph", type=int, default=1,
                     help="also time the step captured as one HIP graph (value = the faster "
                          "of eager and graph, the graph only if its replays match eager)")
@@ -597,6 +623,11 @@ def main():
     if args.tight:
         tight = time_tight(renderer, step, args.steps)
 
+    # the same step fed the raw parameters (activations fused into preprocess)
+    raw = None
+    if args.raw:
+        raw = time_raw(renderer, scene, cam, targets, args.steps)
+
     graph = None
     if args.graph:
         print("[bench] graphed step ...", file=sys.stderr, flush=True)
@@ -644,7 +675,7 @@ def main():
                    "splats": N, "width": W, "height": H, "visible": Nv, "instances": M,
                    "mpixel_per_s": round(P / 1e6 * 1000.0 / ms, 2),
                    "eager_ms_per_step": round(ms_eager, 4), "graph": graph,
-                   "tight_tiles": tight,
+                   "tight_tiles": tight, "raw_activations": raw,
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -89,18 +89,25 @@ int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
  * normal channel (shortest splat axis facing the camera,
  * examples/supervise/model.py:50-74).  feat[N,C], C = 3 (+1) (+3):
  * rgb | depth | normal.  uv/depth/conic/radius/tiles equal those of the
- * separate ops; cov3d[N,6] may be NULL. */
+ * separate ops; cov3d[N,6] may be NULL.  flags 4 = raw parameters: scale is
+ * log scale, rot an unnormalised quaternion, and the kernel applies exp and
+ * F.normalize (GaussianPointCloud's activations, gaussian_points.py:30-35,
+ * 70-81); with flag 4, opacity_raw[N] (logits) -> opacity_act[N] = sigmoid
+ * (both NULL, or both set). */
 int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, float *uv, float *depth, float *cov3d, float *conic,
-                       int32_t *radius, int32_t *tiles, float *feat, void *stream);
+                       int32_t *radius, int32_t *tiles, float *feat, const float *opacity_raw,
+                       float *opacity_act, void *stream);
 /* Backward: d_uv[N,2], d_depth[N] (may be NULL), d_conic[N,3], d_feat[N,C],
  * each addressed with its row stride (ld_*; e.g. views into the d_splat
  * records of gsr_alpha_blending_bwd) -> d_xyz[N,3], d_scale[N,3], d_rot[N,4],
  * d_shs[N,K,3] (or d_shs[N,1,3] + d_shs_rest[N,K-1,3] when split; all
  * overwritten), d_intr[4], d_extr[12], d_center[3] (accumulated; each may be
- * NULL). */
+ * NULL).  With flag 4 the scale and rotation gradients are w.r.t. the raw
+ * parameters, and d_opacity_raw[N] = d_opacity_act * s (1 - s) when
+ * opacity_raw / d_opacity_act / d_opacity_raw are set (all NULL otherwise). */
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
@@ -108,7 +115,8 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
                        const float *d_depth, const float *d_conic, int ld_conic,
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
-                       float *d_extr, float *d_center, void *stream);
+                       float *d_extr, float *d_center, const float *opacity_raw,
+                       const float *d_opacity_act, float *d_opacity_raw, void *stream);
 
 /* ---- msplat.sort_gaussian (msplat.py:133-135) ----------------------------
  * Two phases around one host read of M (the instance count):

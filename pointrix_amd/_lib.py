@@ -34,10 +34,10 @@ SIGNATURES = {
     "gsr_compute_sh_fwd": (_I, [_P, _I64, _I64, _I64, _P, _I64, _I, _I, _I, _P, _P]),
     "gsr_compute_sh_bwd": (_I, [_P, _I64, _I64, _I64, _P, _I64, _I, _I, _I, _P, _P, _P, _P]),
     "gsr_preprocess_fwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I,
-                                _P, _P, _P, _P, _P, _P, _P, _P]),
+                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_preprocess_bwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I,
                                 _P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
-                                _P]),
+                                _P, _P, _P, _P]),
     "gsr_sort_ws_gauss_bytes": (_SZ, [_I64]),
     "gsr_sort_ws_inst_bytes": (_SZ, [_I64]),
     "gsr_sort_count": (_I, [_P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),

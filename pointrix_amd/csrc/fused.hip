@@ -19,6 +19,24 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kFlagDepth = 1;
 constexpr int kFlagNormal = 2;
+// scale, rotation and opacity arrive as the model's raw parameters (log
+// scale, unnormalised quaternion, opacity logit): the kernels apply exp,
+// normalize and sigmoid (GaussianPointCloud's activations,
+// gaussian_points.py:30-35,70-81) and return gradients w.r.t. the raw values
+constexpr int kFlagRaw = 4;
+
+__device__ __forceinline__ float3 act_scale(float3 s) {
+  return make_float3(expf(s.x), expf(s.y), expf(s.z));
+}
+
+// torch.nn.functional.normalize: q / max(|q|, 1e-12)
+__device__ __forceinline__ float4 act_rot(float4 q, float &norm) {
+  norm = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+  const float d = fmaxf(norm, 1e-12f);
+  return make_float4(q.x / d, q.y / d, q.z / d, q.w / d);
+}
+
+__device__ __forceinline__ float act_opacity(float o) { return 1.0f / (1.0f + expf(-o)); }
 
 // Load the first KE coefficients (x3 channels) of splat i, either from one
 // [N, KS, 3] array or, when rest != nullptr, from the DC array [N, 1, 3] and
@@ -70,9 +88,11 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
     float nearest,
     float extent, int flags, int C, float *__restrict__ uv, float *__restrict__ depth,
     float *__restrict__ cov_out, float *__restrict__ conic, int32_t *__restrict__ radius,
-    int32_t *__restrict__ tiles, float *__restrict__ feat) {
+    int32_t *__restrict__ tiles, float *__restrict__ feat, const float *__restrict__ opac_raw,
+    float *__restrict__ opac_act) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (opac_act != nullptr) opac_act[i] = act_opacity(opac_raw[i]);
   const CamV cam = cam_load(intr, extr);
   const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
   float u, v, d;
@@ -85,8 +105,13 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
   float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int r = 0, t = 0;
   if (d != 0.0f) {
-    const float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-    const float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
+    float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    if (flags & kFlagRaw) {
+      float qn;
+      s = act_scale(s);
+      q = act_rot(q, qn);
+    }
     covariance3d(s, q, cov);
     EwaFwd e;
     ewa_forward(cam, x, cov, W, H, e);
@@ -153,7 +178,9 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     const float *__restrict__ d_depth, const float *__restrict__ d_conic, int ld_conic,
     const float *__restrict__ d_feat, int ld_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
     float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_shs_rest,
-    float *__restrict__ d_intr, float *__restrict__ d_extr, float *__restrict__ d_center) {
+    float *__restrict__ d_intr, float *__restrict__ d_extr, float *__restrict__ d_center,
+    const float *__restrict__ opac_raw, const float *__restrict__ d_opac_act,
+    float *__restrict__ d_opac_raw) {
   constexpr int kRest = (KS - 1) * 3;
   __shared__ __align__(16) float s_rest[KS > 1 ? kThreads * kRest : 4];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,9 +196,18 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
 #pragma unroll
     for (int j = 0; j < KE * 3; ++j) dsh[j] = 0.0f;
     const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    float3 s_act = make_float3(1.f, 1.f, 1.f);
+    float4 q_act = make_float4(1.f, 0.f, 0.f, 0.f);
+    float q_norm = 1.0f;
     if (radius[i] > 0) {
-      const float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-      const float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+      float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
+      float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+      if (flags & kFlagRaw) {
+        s = act_scale(s);
+        q = act_rot(q, q_norm);
+        s_act = s;
+        q_act = q;
+      }
       const float cen[3] = {center[0], center[1], center[2]};
       // colour: SH -> +0.5 -> clamp(min=0) (gradient passes where x >= 0)
       float len;
@@ -242,6 +278,26 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     d_xyz[3 * i] = gx.x;
     d_xyz[3 * i + 1] = gx.y;
     d_xyz[3 * i + 2] = gx.z;
+    if (flags & kFlagRaw) {
+      // through exp: ds_raw = ds * s; through normalize: (g - q (q.g)) / |q|
+      // (1 / 1e-12 where the norm is clamped); through sigmoid: s (1 - s)
+      gs[0] *= s_act.x;
+      gs[1] *= s_act.y;
+      gs[2] *= s_act.z;
+      const float qa[4] = {q_act.x, q_act.y, q_act.z, q_act.w};
+      const float dot = qa[0] * gq[0] + qa[1] * gq[1] + qa[2] * gq[2] + qa[3] * gq[3];
+      if (q_norm > 1e-12f) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gq[k] = (gq[k] - qa[k] * dot) / q_norm;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gq[k] = gq[k] / 1e-12f;
+      }
+      if (d_opac_raw != nullptr) {
+        const float a = act_opacity(opac_raw[i]);
+        d_opac_raw[i] = d_opac_act[i] * (a * (1.0f - a));
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
 #pragma unroll
@@ -320,10 +376,11 @@ int launch_fwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const f
                   const float *rot, const float *shs, const float *shs_rest, int64_t n, const float *intr,
                   const float *extr, const float *center, int W, int H, int gx, int gy,
                   float nearest, float extent, int flags, int C, float *uv, float *depth,
-                  float *cov_out, float *conic, int32_t *radius, int32_t *tiles, float *feat) {
+                  float *cov_out, float *conic, int32_t *radius, int32_t *tiles, float *feat,
+                  const float *opac_raw, float *opac_act) {
 #define GSR_FWD_ARGS                                                                          \
   xyz, scale, rot, shs, shs_rest, n, intr, extr, center, W, H, gx, gy, nearest, extent, flags, \
-      C, uv, depth, cov_out, conic, radius, tiles, feat
+      C, uv, depth, cov_out, conic, radius, tiles, feat, opac_raw, opac_act
   switch (KE) {
     case 1: preprocess_fwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_fwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_FWD_ARGS); break; } return GSR_ERR_ARG;
@@ -343,11 +400,12 @@ int launch_bwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const f
                   int ld_uv, const float *d_depth, const float *d_conic, int ld_conic,
                   const float *d_feat, int ld_feat, float *d_xyz,
                   float *d_scale, float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
-                  float *d_extr, float *d_center) {
+                  float *d_extr, float *d_center,
+                  const float *opac_raw, const float *d_opac_act, float *d_opac_raw) {
 #define GSR_BWD_ARGS                                                                          \
   xyz, scale, rot, shs, shs_rest, n, intr, extr, center, W, H, nearest, extent, flags, C,      \
       radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot,   \
-      d_shs, d_shs_rest, d_intr, d_extr, d_center
+      d_shs, d_shs_rest, d_intr, d_extr, d_center, opac_raw, d_opac_act, d_opac_raw
   switch (KE) {
     case 1: preprocess_bwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_bwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
@@ -377,9 +435,11 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
                        const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
                        int flags, float *uv, float *depth, float *cov3d, float *conic,
-                       int32_t *radius, int32_t *tiles, float *feat, void *stream) {
+                       int32_t *radius, int32_t *tiles, float *feat, const float *opacity_raw,
+                       float *opacity_act, void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
-                  (flags & ~3) == 0,
+                  (flags & ~7) == 0 && ((opacity_raw == nullptr) == (opacity_act == nullptr)) &&
+                  (opacity_raw == nullptr || (flags & kFlagRaw)),
               "preprocess_fwd: bad arguments (K=%d flags=%d)", K, flags);
   if (n == 0) return GSR_OK;
   const int gx = (W + 15) / 16, gy = (H + 15) / 16;
@@ -389,7 +449,7 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
   int rc;
 #define GSR_ARGS                                                                              \
   KE, nb, st, xyz, scale, rot, shs, shs_rest, n, intr, extr, cam_center, W, H, gx, gy, nearest, \
-      extent, flags, C, uv, depth, cov3d, conic, radius, tiles, feat
+      extent, flags, C, uv, depth, cov3d, conic, radius, tiles, feat, opacity_raw, opacity_act
   switch (K) {
     case 1: rc = launch_fwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_fwd_ks<4>(GSR_ARGS); break;
@@ -408,9 +468,13 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
                        const float *d_depth, const float *d_conic, int ld_conic,
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
-                       float *d_extr, float *d_center, void *stream) {
+                       float *d_extr, float *d_center, const float *opacity_raw,
+                       const float *d_opacity_act, float *d_opacity_raw, void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
-                  (flags & ~3) == 0,
+                  (flags & ~7) == 0 &&
+                  ((opacity_raw == nullptr) == (d_opacity_raw == nullptr)) &&
+                  ((d_opacity_raw == nullptr) || (d_opacity_act != nullptr)) &&
+                  (opacity_raw == nullptr || (flags & kFlagRaw)),
               "preprocess_bwd: bad arguments (K=%d flags=%d)", K, flags);
   if (n == 0) return GSR_OK;
   const int KE = effective_k(K, degree), C = feat_channels(flags);
@@ -420,7 +484,7 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
 #define GSR_ARGS                                                                               \
   KE, nb, st, xyz, scale, rot, shs, shs_rest, n, intr, extr, cam_center, W, H, nearest, extent, \
       flags, C, radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, \
-      d_rot, d_shs, d_shs_rest, d_intr, d_extr, d_center
+      d_rot, d_shs, d_shs_rest, d_intr, d_extr, d_center, opacity_raw, d_opacity_act, d_opacity_raw
   switch (K) {
     case 1: rc = launch_bwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_bwd_ks<4>(GSR_ARGS); break;

@@ -436,6 +436,7 @@ def compute_sh(shs, dirs, degree=None):
 # ---------------------------------------------------------------------------
 FLAG_DEPTH = 1
 FLAG_NORMAL = 2
+FLAG_RAW = 4
 
 
 def _rows(t: torch.Tensor) -> torch.Tensor:
@@ -454,9 +455,10 @@ def _aligned16(t: torch.Tensor) -> torch.Tensor:
 class _Preprocess(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, scale, rot, shs, shs_rest, intr, extr, center, W, H, degree, flags,
-                nearest, extent, keep_cov):
-        _dev_check(xyz, scale, rot, shs, shs_rest, intr, extr, center)
+                nearest, extent, keep_cov, opacity_raw):
+        _dev_check(xyz, scale, rot, shs, shs_rest, intr, extr, center, opacity_raw)
         x, s, q = _f32(xyz), _f32(scale), _f32(rot)
+        o_raw = None if opacity_raw is None else _f32(opacity_raw).reshape(-1)
         if shs_rest is None:
             sh, rest = _aligned16(_f32(shs)), None
             K = sh.shape[1]
@@ -479,20 +481,25 @@ class _Preprocess(torch.autograd.Function):
         tiles = torch.empty(n, dtype=torch.int32, device=dev)
         feat = torch.empty(n, C, device=dev)
         cov = torch.empty(n, 6, device=dev) if keep_cov else torch.empty(0, device=dev)
+        if o_raw is not None and o_raw.shape[0] != n:
+            raise ValueError("opacity must hold one logit per splat")
+        o_act = torch.empty(n, 1, device=dev) if o_raw is not None else torch.empty(0, device=dev)
         with _span("preprocess_fwd", dev):
             call("gsr_preprocess_fwd", ptr(x), ptr(s), ptr(q), ptr(sh), ptr(rest), n, K,
                  int(degree), ptr(k), ptr(e), ptr(c), int(W), int(H), float(nearest),
                  float(extent), int(flags), ptr(uv), ptr(depth), ptr(cov) if keep_cov else None,
-                 ptr(conic), ptr(radius), ptr(tiles), ptr(feat), stream_of(x))
-        ctx.save_for_backward(x, s, q, sh, rest, k, e, c, radius)
+                 ptr(conic), ptr(radius), ptr(tiles), ptr(feat), ptr(o_raw),
+                 ptr(o_act) if o_raw is not None else None, stream_of(x))
+        ctx.save_for_backward(x, s, q, sh, rest, k, e, c, radius, o_raw)
         ctx.cfg = (int(W), int(H), int(degree), int(flags), float(nearest), float(extent), intr,
                    extr, center, C, K)
+        ctx.o_shape = None if opacity_raw is None else opacity_raw.shape
         ctx.mark_non_differentiable(radius, tiles, cov)
-        return uv, depth, conic, radius, tiles, feat, cov
+        return uv, depth, conic, radius, tiles, feat, cov, o_act
 
     @staticmethod
-    def backward(ctx, g_uv, g_depth, g_conic, _gr, _gt, g_feat, _gc):
-        x, s, q, sh, rest, k, e, c, radius = ctx.saved_tensors
+    def backward(ctx, g_uv, g_depth, g_conic, _gr, _gt, g_feat, _gc, g_oact):
+        x, s, q, sh, rest, k, e, c, radius, o_raw = ctx.saved_tensors
         W, H, degree, flags, nearest, extent, intr, extr, center, C, K = ctx.cfg
         n = x.shape[0]
         dev = x.device
@@ -509,21 +516,27 @@ class _Preprocess(torch.autograd.Function):
         d_k = z(4) if need_k else None
         d_e = z(12) if need_e else None
         d_c = z(3) if need_c else None
+        d_o = None
+        if o_raw is not None and ctx.needs_input_grad[15]:
+            g_oact = z(n) if g_oact is None else _f32(g_oact).reshape(-1)
+            d_o = torch.empty_like(o_raw)
         with _span("preprocess_bwd", dev):
             call("gsr_preprocess_bwd", ptr(x), ptr(s), ptr(q), ptr(sh), ptr(rest), n, K, degree,
                  ptr(k), ptr(e), ptr(c), W, H, nearest, extent, flags, ptr(radius), ptr(g_uv),
                  g_uv.stride(0), ptr(g_depth), ptr(g_conic), g_conic.stride(0), ptr(g_feat),
                  g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_rest),
-                 ptr(d_k), ptr(d_e), ptr(d_c), stream_of(x))
+                 ptr(d_k), ptr(d_e), ptr(d_c), ptr(o_raw) if d_o is not None else None,
+                 ptr(g_oact) if d_o is not None else None, ptr(d_o), stream_of(x))
         return (d_x, d_s, d_q, d_sh, d_rest, _intr_grad(d_k, intr) if need_k else None,
                 _extr_grad(d_e, extr) if need_e else None,
                 d_c.reshape(center.shape).to(center.dtype) if need_c else None,
-                None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None,
+                None if d_o is None else d_o.view(ctx.o_shape))
 
 
 def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_matrix,
                camera_center, W, H, degree=None, depth_channel=False, normal_channel=False,
-               nearest=0.2, extent=1.3, keep_cov=False):
+               nearest=0.2, extent=1.3, keep_cov=False, raw=False, opacity=None):
     """Everything render_iter does per splat before sorting, in one kernel.
 
     Inputs as MsplatRender.render_iter receives them (activated scaling,
@@ -533,12 +546,26 @@ def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_mat
     feat = clamp(SH colour + 0.5, 0) | depth | camera-space normal.  uv,
     depth, conic, radius and tiles equal those of project_point +
     compute_cov3d + ewa_project.
+
+    ``raw=True``: scaling, rotation (and ``opacity``, if given) are the
+    model's raw parameters -- log scale, unnormalised quaternion, opacity
+    logit -- and the kernel applies GaussianPointCloud's activations (exp,
+    F.normalize, sigmoid; gaussian_points.py:30-35,70-81) itself, so the
+    gradients land on the raw parameters without separate elementwise
+    passes.  With ``opacity`` the activated opacity [N,1] is appended to the
+    returned tuple.
     """
     flags = (FLAG_DEPTH if depth_channel else 0) | (FLAG_NORMAL if normal_channel else 0)
+    if raw:
+        flags |= FLAG_RAW
+    elif opacity is not None:
+        raise ValueError("preprocess: opacity logits are taken only with raw=True")
     shs_dc, shs_rest = (shs[0], shs[1]) if isinstance(shs, (tuple, list)) else (shs, None)
-    return _Preprocess.apply(position, scaling, rotation, shs_dc, shs_rest, intrinsic_params,
-                             extrinsic_matrix, camera_center, W, H,
-                             -1 if degree is None else degree, flags, nearest, extent, keep_cov)
+    out = _Preprocess.apply(position, scaling, rotation, shs_dc, shs_rest, intrinsic_params,
+                            extrinsic_matrix, camera_center, W, H,
+                            -1 if degree is None else degree, flags, nearest, extent, keep_cov,
+                            opacity)
+    return out if opacity is not None else out[:7]
 
 
 # ---------------------------------------------------------------------------

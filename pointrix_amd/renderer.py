@@ -134,16 +134,24 @@ class MsplatRender:
         self.intermediates = None
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
-                    position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
+                    position, opacity, scaling, rotation, shs, normals=None, raw=False,
+                    **kwargs) -> dict:
         """One view (msplat.py:51-158).  ``shs`` is [N,K,3], or the pair
         (features [N,1,3], features_rest [N,K-1,3]) of pointrix's point cloud,
-        which the fused path reads without concatenating."""
+        which the fused path reads without concatenating.  ``raw=True``:
+        opacity, scaling and rotation are the point cloud's raw parameters
+        (logit, log scale, unnormalised wxyz); the fused path applies sigmoid,
+        exp and normalize inside its preprocess kernel (gaussian_points.py:
+        30-35,70-81), the unfused path with torch."""
         if isinstance(shs, (tuple, list)) and not self.cfg.fused:
             shs = torch.cat(list(shs), dim=1)
         if self.cfg.fused:
             return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
                                            camera_center, position, opacity, scaling, rotation,
-                                           shs, normals)
+                                           shs, normals, raw)
+        if raw:
+            opacity, scaling = torch.sigmoid(opacity), torch.exp(scaling)
+            rotation = torch.nn.functional.normalize(rotation, dim=-1)
         direction = position - camera_center.reshape(1, 3)
         direction = direction / direction.norm(dim=1, keepdim=True)
         # SH warm-up: the first (sh_degree+1)^2 coefficients (msplat.py:98-101)
@@ -188,12 +196,17 @@ class MsplatRender:
                 "visibility": radius > 0, "radii": radius}
 
     def _render_iter_fused(self, height, width, extrinsic_matrix, intrinsic_params,
-                           camera_center, position, opacity, scaling, rotation, shs, normals):
+                           camera_center, position, opacity, scaling, rotation, shs, normals,
+                           raw=False):
         normal_in_kernel = normals is None and self.cfg.render_normal
-        uv, depth, conic, radius, tiles, feat, cov3d = ops.preprocess(
+        out = ops.preprocess(
             position, scaling, rotation, shs, intrinsic_params, extrinsic_matrix, camera_center,
             width, height, degree=self.sh_degree, depth_channel=self.cfg.render_depth,
-            normal_channel=normal_in_kernel, keep_cov=self.keep_intermediates)
+            normal_channel=normal_in_kernel, keep_cov=self.keep_intermediates, raw=raw,
+            opacity=opacity if raw else None)
+        uv, depth, conic, radius, tiles, feat, cov3d = out[:7]
+        if raw:
+            opacity = out[7]
         names = [("rgb", 3)] + ([("depth", 1)] if self.cfg.render_depth else []) + \
             ([("normal", 3)] if normal_in_kernel else [])
         if normals is not None:

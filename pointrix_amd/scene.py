@@ -103,17 +103,22 @@ class GaussianScene:
         ps = [p.detach().to(device).requires_grad_(requires_grad) for p in self.params()]
         return GaussianScene(*ps)
 
-    def render_inputs(self, split_sh: bool = False) -> dict:
+    def render_inputs(self, split_sh: bool = False, raw: bool = False) -> dict:
         """Activated tensors, as BaseModel.forward passes them (base_model.py:66-71).
 
         split_sh: pass SH as the pair (features, features_rest) instead of
-        their concatenation (get_shs, gaussian_points.py:86-90)."""
+        their concatenation (get_shs, gaussian_points.py:86-90).
+        raw: pass opacity, scaling and rotation unactivated with ``raw=True``,
+        for a renderer that applies the activations in its own kernel."""
         if self.features_rest is None or self.features_rest.shape[1] == 0:
             shs = self.features
         elif split_sh:
             shs = (self.features, self.features_rest)
         else:
             shs = torch.cat([self.features, self.features_rest], dim=1)
+        if raw:
+            return {"position": self.position, "opacity": self.opacity,
+                    "scaling": self.scaling, "rotation": self.rotation, "shs": shs, "raw": True}
         return {"position": self.position,
                 "opacity": torch.sigmoid(self.opacity),
                 "scaling": torch.exp(self.scaling),
