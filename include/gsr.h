@@ -303,6 +303,36 @@ int gsr_gather_rows(const GsrRowCopy *table, int ntab, const int32_t *index, int
 int gsr_uv_grad_scale(const float *d_splat, int ld, int64_t n, float sx, float sy, float *out,
                       void *stream);
 
+/* ---- camera pose (camera_model.py:92-175, utils/pose.py:40-84) -----------
+ * q[C,4] wxyz (normalised inside as q / max(|q|, 1e-12)), t[C,3] ->
+ * E[C,4,4] = [R | t; 0 0 0 1] and center[C,3] = -R^T t (may be NULL).
+ * Backward: dE[C,4,4] and dcenter[C,3] (either may be NULL) -> dq[C,4],
+ * dt[C,3] (overwritten). */
+int gsr_camera_pose_fwd(const float *q, const float *t, int C, float *E, float *center,
+                        void *stream);
+int gsr_camera_pose_bwd(const float *q, const float *t, int C, const float *dE,
+                        const float *dcenter, float *dq, float *dt, void *stream);
+
+/* ---- output planes (RenderFeatures.split, renderer_utils.py:57-71, and the
+ * normal mapping of examples/supervise/renderer.py:89-96) -------------------
+ * in[C,P] (P = H*W) -> out[C,P]: planes copied, except the three planes from
+ * normal_first (< 0: none), which become (n / |n| + 1) / 2 with |n| over the
+ * three.  Backward: the gradient of each output group (channels first ..
+ * first+count-1, groups in channel order covering all C; grad NULL = zero)
+ * -> d_in[C,P] (overwritten). */
+#define GSR_MAX_PLANE_GROUPS 8
+typedef struct {
+  const float *grad[GSR_MAX_PLANE_GROUPS];
+  int first[GSR_MAX_PLANE_GROUPS];
+  int count[GSR_MAX_PLANE_GROUPS];
+  int ngroups;
+} GsrPlaneGrads;
+int gsr_plane_grads_bytes(void); /* sizeof(GsrPlaneGrads), for bindings */
+int gsr_planes_fwd(const float *in, int C, int64_t P, int normal_first, float *out,
+                   void *stream);
+int gsr_planes_bwd(const float *in, int C, int64_t P, int normal_first,
+                   const GsrPlaneGrads *grads, float *d_in, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ import torch
 from . import _lib, ops
 from ._lib import call, ptr, stream_of
 from .graph import CapturedStep
+from .renderer import _split_planes
 
 TILE_SHAPES = ((16, 16), (8, 32), (32, 8))
 BATCH_DEPTHS = (256, 128, 64)
@@ -283,14 +284,7 @@ class BatchedRender:
         Ts = _Unpack.apply(T[None], lay) if self.render_opacity else None
         views = []
         for v in range(lay.V):
-            img = imgs[v]
-            d, start = {}, 0
-            for name, c in self.channel_names():
-                d[name] = img[start:start + c]
-                start += c
-            if "normal" in d:
-                nrm = d["normal"]
-                d["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+            d = _split_planes(imgs[v], self.channel_names())
             if Ts is not None:
                 d["opacity"] = 1.0 - Ts[v]
             views.append(d)

@@ -35,15 +35,31 @@ struct TileView {
   int64_t op_offset;
 };
 
-// LDS of one tile: a batch of BD splats (BD <= TW*TH; the "batch depth").
+constexpr int kMaxTileWaves = 4;  // 256-pixel tiles, one wave per 64 pixels
+
+// LDS of one tile: a batch of BD splats (BD <= TW*TH; the "batch depth"), and
+// for the forward the per-wave lists of batch slots that can reach the wave's
+// pixels (slot indices in depth order) with their per-loading-wave counts.
 template <int C, int BD>
 struct BlendLds {
   SplatLds sp[BD];
   float f[C * BD];
   int id[BD];
+  int wcnt[kMaxTileWaves][kMaxTileWaves];  // [strip][loading wave]
+  uint8_t list[kMaxTileWaves][BD];
   int misc;
 };
 
+// Forward of one tile.  Each wave owns a strip of 64 / TW pixel rows across
+// the tile.  When a batch is staged, the thread loading a splat also tests it
+// against every strip with the opacity-aware reachability test of the
+// binning (tile_reachable: never false where a pixel of the box reaches
+// alpha >= kAlphaMin), and the batch slots are compacted into one depth-ordered
+// list per strip.  A wave then walks only its list: a splat left out could not
+// have changed any of its pixels (it is neither a contributor nor the splat at
+// which a pixel stops, both of which need alpha >= kAlphaMin), so every
+// pixel's colour, T and contributor count are exactly those of the walk over
+// the whole range.
 template <int C, int TW, int TH, int BD>
 __device__ __forceinline__ void blend_fwd_tile(
     const TileView &tv, int start, int end, const float *__restrict__ uv,
@@ -52,12 +68,22 @@ __device__ __forceinline__ void blend_fwd_tile(
     float *__restrict__ out, float *__restrict__ Tfin, int32_t *__restrict__ ncon,
     BlendLds<C, BD> &sm) {
   constexpr int B = TW * TH;
+  constexpr int NW = B / 64;      // waves (strips) per tile
+  constexpr int RW = 64 / TW;     // pixel rows per strip
   static_assert(BD <= B, "batch depth exceeds the workgroup");
+  static_assert(BD <= 256, "batch slots are stored as bytes");
+  static_assert(64 % TW == 0 && B % 64 == 0 && NW <= kMaxTileWaves, "tile shape");
   const int tx = tv.tile % tv.gx, ty = tv.tile / tv.gx;
   const int px = tx * TW + (int)threadIdx.x % TW;
   const int py = ty * TH + (int)threadIdx.x / TW;
   const bool inside = px < tv.W && py < tv.H;
   const float fpx = (float)px, fpy = (float)py;
+  const int wave = (int)threadIdx.x >> 6;
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  // pixel-centre box of the tile (clipped to the image) and of each strip
+  const float bx0 = (float)(tx * TW), bx1 = (float)(min(tx * TW + TW, tv.W) - 1);
+  const int by0 = ty * TH, byl = min(ty * TH + TH, tv.H) - 1;
   float T = 1.0f;
   float acc[C];
 #pragma unroll
@@ -67,24 +93,57 @@ __device__ __forceinline__ void blend_fwd_tile(
   for (int base = start; base < end; base += BD) {
     if (__syncthreads_count(done) == B) break;
     const int j = base + (int)threadIdx.x;
+    unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < end) {
       const int g = ids[j];
+      const float u = uv[2 * (int64_t)g], v = uv[2 * (int64_t)g + 1];
+      const float ca = conic[3 * (int64_t)g], cb = conic[3 * (int64_t)g + 1],
+                  cc = conic[3 * (int64_t)g + 2];
+      const float o = opacity[g - tv.op_offset];
       SplatLds sp;
-      sp.xy = make_float2(uv[2 * (int64_t)g], uv[2 * (int64_t)g + 1]);
-      sp.co = make_float4(-0.5f * conic[3 * (int64_t)g], -conic[3 * (int64_t)g + 1],
-                          -0.5f * conic[3 * (int64_t)g + 2], opacity[g - tv.op_offset]);
+      sp.xy = make_float2(u, v);
+      sp.co = make_float4(-0.5f * ca, -cb, -0.5f * cc, o);
       sm.sp[threadIdx.x] = sp;
 #pragma unroll
       for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
+      bool reach;
+      const TightSplat ts = tight_setup(u, v, ca, cb, cc, o, reach);
+      if (reach) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const int y0 = by0 + w * RW, y1 = min(y0 + RW - 1, byl);
+          if (y0 <= y1 && tile_reachable(ts, bx0, bx1, (float)y0, (float)y1)) mask |= 1u << w;
+        }
+      }
+    }
+    // compaction: slot t goes to position (slots before t in strip w) of list w
+    int pos[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint64_t bal = __ballot((mask >> w) & 1u);
+      pos[w] = (int)__popcll(bal & lt);
+      if (lane == 0) sm.wcnt[w][wave] = (int)__popcll(bal);
     }
     __syncthreads();
-    const int cnt = min(BD, end - base);
-    for (int k = 0; k < cnt; ++k) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if ((mask >> w) & 1u) {
+        int off = pos[w];
+        for (int q = 0; q < wave; ++q) off += sm.wcnt[w][q];
+        sm.list[w][off] = (uint8_t)threadIdx.x;
+      }
+    }
+    int n_mine = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) n_mine += sm.wcnt[wave][q];
+    __syncthreads();
+    for (int i = 0; i < n_mine; ++i) {
       // Branch-light walk: lanes keep their own state through selects; the
       // wave only branches (uniformly) to skip a splat no active lane needs
       // or to leave once every lane is done.  Decisions are exactly those of
       // the straightforward loop (a lane that skips or stops changes nothing).
       if (__ballot(!done) == 0ull) break;
+      const int k = __builtin_amdgcn_readfirstlane((int)sm.list[wave][i]);
       const SplatLds sp = sm.sp[k];
       const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
       const float power = splat_power(sp.co, dx, dy);
@@ -166,9 +225,12 @@ __device__ __forceinline__ void blend_bwd_tile(
 #pragma unroll
   for (int c = 0; c < C; ++c) gsum += gpix[c];
   const float bg_term = bg * gsum + gT;  // d out / d T_final summed with d T_final
-  float accum[C], last_f[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) accum[c] = last_f[c] = 0.0f;
+  // dL/dalpha_i = T_i sum_c (f_ic - A_ic) g_c - ..., where A_c (the colour
+  // blended behind splat i) obeys A <- a' f' + (1 - a') A over the splats
+  // behind it (a', f' the next splat back).  The recurrence is linear, so
+  // the pixel keeps only the scalars S = sum_c A_c g_c and F = sum_c f'_c g_c
+  // instead of 2C per-channel values.
+  float S = 0.0f, F = 0.0f;
   float last_alpha = 0.0f;
 
   for (int bend = stop; bend > start; bend -= BD) {
@@ -210,17 +272,15 @@ __device__ __forceinline__ void blend_bwd_tile(
         const float Tn = T * inv_om;
         T = contrib ? Tn : T;
         const float w = contrib ? alpha * Tn : 0.0f;
-        float dL_dalpha = 0.0f;
+        float fg = sm.f[k] * gpix[0];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const float f = sm.f[c * BD + k];
-          v[6 + c] = w * gpix[c];
-          const float acc_n = last_alpha * last_f[c] + (1.0f - last_alpha) * accum[c];
-          accum[c] = contrib ? acc_n : accum[c];
-          last_f[c] = contrib ? f : last_f[c];
-          dL_dalpha += (f - acc_n) * gpix[c];
-        }
-        dL_dalpha *= Tn;
+        for (int c = 1; c < C; ++c) fg = fmaf(sm.f[c * BD + k], gpix[c], fg);
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[6 + c] = w * gpix[c];
+        const float S_n = fmaf(last_alpha, F, (1.0f - last_alpha) * S);
+        S = contrib ? S_n : S;
+        F = contrib ? fg : F;
+        float dL_dalpha = (fg - S_n) * Tn;
         dL_dalpha -= T_final * inv_om * bg_term;
         last_alpha = contrib ? alpha : last_alpha;
         // alpha = min(0.99, o G): no gradient where clamped

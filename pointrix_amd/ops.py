@@ -10,6 +10,8 @@ GPU -- there is no CPU path.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _lib
@@ -645,3 +647,97 @@ def l1_ssim_loss(pred, gt, lambda_ssim=0.2):
     ssim_loss = 1.0 - s[0]
     loss = (1.0 - lambda_ssim) * s[1] + lambda_ssim * ssim_loss
     return {"loss": loss, "L1_loss": s[1], "ssim_loss": ssim_loss}
+
+
+# ---------------------------------------------------------------------------
+# Camera pose (camera_model.py:92-175 with utils/pose.py:40-84)
+# ---------------------------------------------------------------------------
+class _CameraPose(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, q, t):
+        _dev_check(q, t)
+        qc, tc = _f32(q).reshape(-1, 4), _f32(t).reshape(-1, 3)
+        C = qc.shape[0]
+        if tc.shape[0] != C:
+            raise ValueError("camera_pose: one translation per quaternion")
+        E = torch.empty(C, 4, 4, device=qc.device)
+        center = torch.empty(C, 3, device=qc.device)
+        call("gsr_camera_pose_fwd", ptr(qc), ptr(tc), C, ptr(E), ptr(center), stream_of(qc))
+        ctx.save_for_backward(qc, tc)
+        ctx.shapes = (q.shape, t.shape)
+        ctx.set_materialize_grads(False)
+        return E, center
+
+    @staticmethod
+    def backward(ctx, gE, gc):
+        qc, tc = ctx.saved_tensors
+        dq, dt = torch.empty_like(qc), torch.empty_like(tc)
+        gE = None if gE is None else _f32(gE)
+        gc = None if gc is None else _f32(gc)
+        call("gsr_camera_pose_bwd", ptr(qc), ptr(tc), qc.shape[0], ptr(gE), ptr(gc), ptr(dq),
+             ptr(dt), stream_of(qc))
+        return dq.view(ctx.shapes[0]), dt.view(ctx.shapes[1])
+
+
+def camera_pose(q, t):
+    """pointrix's CameraModel composition in one kernel (and one backward):
+    wxyz quaternion(s) q [...,4] (normalised as F.normalize does) and
+    translation(s) t [...,3] -> (world-to-camera E [...,4,4] = [R | t; 0 0 0 1],
+    camera centre [...,3] = -R^T t), differentiable in q and t."""
+    lead = q.shape[:-1]
+    E, c = _CameraPose.apply(q, t)
+    return E.view(*lead, 4, 4), c.view(*lead, 3)
+
+
+# ---------------------------------------------------------------------------
+# Output planes (RenderFeatures.split, renderer_utils.py:57-71; the normal
+# mapping of examples/supervise/renderer.py:89-96)
+# ---------------------------------------------------------------------------
+class _PlaneGrads(ctypes.Structure):
+    _fields_ = [("grad", ctypes.c_void_p * 8), ("first", ctypes.c_int * 8),
+                ("count", ctypes.c_int * 8), ("ngroups", ctypes.c_int)]
+
+
+class _Planes(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rendered, sizes, normal_first):
+        _dev_check(rendered)
+        x = _f32(rendered)
+        C, P = x.shape[0], x[0].numel()
+        out = torch.empty_like(x)
+        call("gsr_planes_fwd", ptr(x), C, P, normal_first, ptr(out), stream_of(x))
+        ctx.save_for_backward(x)
+        ctx.cfg = (tuple(sizes), normal_first)
+        ctx.set_materialize_grads(False)
+        return tuple(torch.split(out, list(sizes), dim=0))
+
+    @staticmethod
+    def backward(ctx, *grads):
+        x, = ctx.saved_tensors
+        sizes, nf = ctx.cfg
+        C, P = x.shape[0], x[0].numel()
+        tab, keep, first = _PlaneGrads(), [], 0
+        for k, (c, g) in enumerate(zip(sizes, grads)):
+            if g is not None:
+                g = _f32(g)
+                keep.append(g)
+            tab.grad[k] = None if g is None else g.data_ptr()
+            tab.first[k] = first
+            tab.count[k] = c
+            first += c
+        tab.ngroups = len(sizes)
+        d = torch.empty_like(x)
+        call("gsr_planes_bwd", ptr(x), C, P, nf, ctypes.byref(tab), ptr(d), stream_of(x))
+        return d, None, None
+
+
+def output_planes(rendered, sizes, normal_first=-1):
+    """Split the blended image [C,H,W] into planes of ``sizes`` (channel
+    order), the group starting at channel ``normal_first`` (three planes;
+    -1 for none) mapped to (n / |n| + 1) / 2 -- as MsplatRender's
+    post-processing does -- in one kernel, with a one-kernel backward that
+    assembles the image gradient from the groups' gradients."""
+    sizes = [int(c) for c in sizes]
+    if sum(sizes) != rendered.shape[0] or len(sizes) > 8:
+        raise ValueError(f"output_planes: sizes {sizes} do not tile {rendered.shape[0]} channels")
+    return _Planes.apply(rendered, sizes, int(normal_first))

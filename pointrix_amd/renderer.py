@@ -226,13 +226,7 @@ class MsplatRender:
                                   "radius": radius, "ids": ids, "tile_range": tile_range,
                                   "features": feat, "rendered": rendered, "T_final": T_final,
                                   "n_contrib": n_contrib}
-        split, start = {}, 0
-        for name, c in names:
-            split[name] = rendered[start:start + c]
-            start += c
-        if "normal" in split:
-            nrm = split["normal"]
-            split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        split = _split_planes(rendered, names)
         if self.cfg.render_opacity:
             split["opacity"] = (1.0 - T_final)[None]
         return {"rendered_features_split": split, "uv_points": ndc,
@@ -266,6 +260,33 @@ class MsplatRender:
 
     def state_dict(self):
         return {"sh_degree": self.sh_degree}
+
+
+def _split_planes(rendered, names):
+    """{name: planes} of the blended image in channel order, the "normal"
+    group mapped to [0,1] as (n / |n| + 1) / 2 (examples/supervise/
+    renderer.py:89-96): one fused kernel each way when there is more than one
+    group, a plain view for a single group."""
+    if len(names) == 1 and names[0][0] != "normal":
+        return {names[0][0]: rendered}
+    first, nf = 0, -1
+    for name, c in names:
+        if name == "normal":
+            if c != 3:
+                nf = -2
+                break
+            nf = first
+        first += c
+    if nf == -2 or len(names) > 8:
+        split, start = {}, 0
+        for name, c in names:
+            split[name] = rendered[start:start + c]
+            start += c
+        nrm = split["normal"]
+        split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        return split
+    planes = ops.output_planes(rendered, [c for _, c in names], nf)
+    return {name: p for (name, _), p in zip(names, planes)}
 
 
 @RENDERER_REGISTRY.register()

@@ -38,3 +38,12 @@ def test_version_and_workspace_sizes():
     b = lib.gsr_sort_ws_gauss_bytes(100000)
     assert 0 < a < b
     assert lib.gsr_sort_ws_inst_bytes(0) > 0
+
+
+def test_struct_layouts_match_bindings():
+    """The ctypes mirrors of the table structs have the library's sizes."""
+    from pointrix_amd import densify, ops, optim
+    lib = _lib.load()
+    assert lib.gsr_adam_entry_bytes() == ctypes.sizeof(optim._Entry)
+    assert lib.gsr_plane_grads_bytes() == ctypes.sizeof(ops._PlaneGrads)
+    assert lib.gsr_row_copy_bytes() == ctypes.sizeof(densify._RowCopy)
