@@ -183,7 +183,8 @@ int gsr_mv_preprocess_bwd(const float *xyz, const float *scale, const float *rot
                           int ld, float *d_xyz, float *d_scale, float *d_rot, float *d_opacity,
                           float *d_shs, float *d_shs_rest, float *d_view_f, void *stream);
 /* conic [V*N,3] / opacity [N]: both NULL for the exact tile contract, both
- * set for tight binning (see gsr_sort_count_tight). */
+ * set for tight binning (see gsr_sort_count_tight).  Every view's tile grid
+ * (view_i columns 2, 3) must be under 65536 tiles per side. */
 int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
                       const float *conic, const float *opacity, int64_t n_per_view, int V,
                       const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,

@@ -42,6 +42,8 @@ class ViewLayout:
         self.V = len(self.widths)
         self.grid = [((w + self.tile_w - 1) // self.tile_w, (h + self.tile_h - 1) // self.tile_h)
                      for w, h in zip(self.widths, self.heights)]
+        if any(gx >= 65536 or gy >= 65536 for gx, gy in self.grid):
+            raise ValueError("a view's tile grid must be under 65536 tiles per side")
         self.tile_off, self.pix_off = [], []
         t = p = 0
         for (gx, gy), w, h in zip(self.grid, self.widths, self.heights):

@@ -415,20 +415,28 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
   return inc - v;
 }
 
+// Per splat: tile count of its rectangle, and the rectangle packed as
+// (x0 | y0 << 16, w | h << 16) for the emission (which then gathers 8 bytes
+// per splat instead of its position and radius).
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
                                                          const int32_t *__restrict__ radius,
                                                          uint32_t n, Grid g,
-                                                         uint32_t *__restrict__ cnt) {
+                                                         uint32_t *__restrict__ cnt,
+                                                         uint2 *__restrict__ rects) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const int r = radius[i];
   uint32_t c = 0;
+  uint2 packed = make_uint2(0u, 0u);
   if (r > 0) {
     const ItemGrid ig = item_grid(g, i);
     const Rect q = tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
-    c = (uint32_t)((q.x1 - q.x0) * (q.y1 - q.y0));
+    const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
+    c = w * h;
+    packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
   }
   cnt[i] = c;
+  rects[i] = packed;
 }
 
 // Tight counts: per splat the number of rectangle tiles tile_reachable
@@ -492,35 +500,38 @@ struct LoadSortedCount {
   __device__ uint32_t operator()(uint32_t s) const { return cnt[ids[s]]; }
 };
 
-// Emission, wave-cooperative: lane l of a wave loads splat (wave base + l);
-// the wave then walks its 64 splats in order and, for each, all 64 lanes
-// write consecutive instances (coalesced stores, no per-splat imbalance).
-// Row/column of instance k within the w-wide rectangle: q = trunc((k + 0.5) /
-// w) in float is exact while k < 2^22 / w (k is never within 0.5/w of a
-// multiple of w), far beyond the largest tile grid.
+// Emission, wave-cooperative: lane l of a wave loads depth-sorted splat
+// (wave base + l) -- its tile rectangle as the count kernel packed it, one
+// 8-byte gather per splat; the wave then walks its 64 splats in order and,
+// for each, all 64 lanes write consecutive instances (coalesced stores, no
+// per-splat imbalance).  Row/column of instance k within the w-wide
+// rectangle: q = trunc((k + 0.5) / w) in float is exact while k < 2^22 / w
+// (k is never within 0.5/w of a multiple of w), far beyond the largest tile
+// grid.
 __global__ __launch_bounds__(kThreads) void emit_kernel(
-    const float *__restrict__ uv, const int32_t *__restrict__ radius,
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
-    const uint32_t *nv_ptr, uint32_t cap, Grid g, uint32_t cap_inst,
-    uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
+    const uint2 *__restrict__ rects, const uint32_t *nv_ptr, uint32_t cap, Grid g,
+    uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
   const uint32_t nv = count_of(nv_ptr, cap);
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
   if (wave_base >= nv) return;  // uniform per wave
   const int lane = lane_id();
   const uint32_t s = wave_base + lane;
   int id = 0, x0 = 0, y0 = 0, w = 1, cnt = 0, gxi = 0, toff = 0;
+  float inv_w = 1.0f;
   uint32_t off = 0;
   if (s < nv) {
     id = sorted_ids[s];
     const ItemGrid ig = item_grid(g, (uint32_t)id);
     gxi = ig.gx;
     toff = ig.toff;
-    const Rect q = tile_rect(uv[2 * (int64_t)id], uv[2 * (int64_t)id + 1], radius[id], ig.gx,
-                             ig.gy, g.tw, g.th);
-    x0 = q.x0;
-    y0 = q.y0;
-    w = max(q.x1 - q.x0, 1);
-    cnt = (q.x1 - q.x0) * (q.y1 - q.y0);
+    const uint2 r = rects[id];
+    x0 = (int)(r.x & 0xFFFFu);
+    y0 = (int)(r.x >> 16);
+    const int rw = (int)(r.y & 0xFFFFu), rh = (int)(r.y >> 16);
+    w = max(rw, 1);
+    inv_w = 1.0f / (float)w;
+    cnt = rw * rh;
     off = offsets[s];
   }
   const int nsplat = min(64, (int)(nv - wave_base));
@@ -532,13 +543,13 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
     const int jcnt = __builtin_amdgcn_readlane(cnt, j);
     const int jgx = __builtin_amdgcn_readlane(gxi, j);
     const int jtoff = __builtin_amdgcn_readlane(toff, j);
+    const float jinv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inv_w), j));
     const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
-    const float inv_w = 1.0f / (float)jw;
     // capacity guard: a caller that sized the instance buffers below the
     // device count (graph replays) gets a truncated, in-bounds emission
     const int kmax = joff >= cap_inst ? 0 : (int)min((uint32_t)jcnt, cap_inst - joff);
     for (int k = lane; k < kmax; k += 64) {
-      const int row = (int)(((float)k + 0.5f) * inv_w);
+      const int row = (int)(((float)k + 0.5f) * jinv);
       const int col = k - row * jw;
       tkeys[joff + k] = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
       tvals[joff + k] = jid;
@@ -713,12 +724,13 @@ int tile_bits(int T) {
 struct GaussWs {
   uint32_t *cnt, *offs, *scalars;  // scalars[0] = Nv, scalars[1] = M
   uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
+  uint2 *rects;                    // exact binning: packed tile rectangles
   SortBufs sb;
 };
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
-  return 2 * align256(c * 4) + align256(64) + align256(c * 8) + sortbufs_bytes(c);
+  return 2 * align256(c * 4) + align256(64) + 2 * align256(c * 8) + sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -728,6 +740,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 4, (void **)&g.offs);
   p = carve(p, 64, (void **)&g.scalars);
   p = carve(p, c * 8, (void **)&g.masks);
+  p = carve(p, c * 8, (void **)&g.rects);
   carve_sortbufs(p, c, g.sb);
 }
 
@@ -754,7 +767,8 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     return gsr_launch_status("sort_count");
   }
   if (gr.conic == nullptr)
-    count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt);
+    count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt,
+                                                              g.rects);
   else
     count_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr,
                                                                     g.cnt, g.masks);
@@ -787,7 +801,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;
   if (gr.conic == nullptr)
-    emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, g.sb.valA, g.offs,
+    emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(g.sb.valA, g.offs, g.rects,
                                                              g.scalars, (uint32_t)n, gr,
                                                              (uint32_t)m, ib.keyA, v_first);
   else
@@ -809,7 +823,8 @@ extern "C" {
 int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *radius,
                          const float *conic, const float *opacity, int64_t n, int W, int H,
                          int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, void *stream) {
-  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0,
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0 &&
+                  (W + tile_w - 1) / tile_w < 65536 && (H + tile_h - 1) / tile_h < 65536,
               "sort_count: bad sizes");
   GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "sort_count: conic without opacity");
   const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0,
