@@ -106,8 +106,9 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
  * d_shs[N,K,3] (or d_shs[N,1,3] + d_shs_rest[N,K-1,3] when split; all
  * overwritten), d_intr[4], d_extr[12], d_center[3] (accumulated; each may be
  * NULL).  With flag 4 the scale and rotation gradients are w.r.t. the raw
- * parameters, and d_opacity_raw[N] = d_opacity_act * s (1 - s) when
- * opacity_raw / d_opacity_act / d_opacity_raw are set (all NULL otherwise). */
+ * parameters, and d_opacity_raw[N] = d_opacity_act[i * ld_opacity_act] *
+ * s (1 - s) when opacity_raw / d_opacity_act / d_opacity_raw are set (all
+ * NULL otherwise; d_opacity_act may be a column of the blend's records). */
 int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,
@@ -116,7 +117,8 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
                        float *d_extr, float *d_center, const float *opacity_raw,
-                       const float *d_opacity_act, float *d_opacity_raw, void *stream);
+                       const float *d_opacity_act, int ld_opacity_act, float *d_opacity_raw,
+                       void *stream);
 
 /* ---- msplat.sort_gaussian (msplat.py:133-135) ----------------------------
  * Two phases around one host read of M (the instance count):

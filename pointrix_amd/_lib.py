@@ -37,7 +37,7 @@ SIGNATURES = {
                                 _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_preprocess_bwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I,
                                 _P, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
-                                _P, _P, _P, _P]),
+                                _P, _P, _I, _P, _P]),
     "gsr_sort_ws_gauss_bytes": (_SZ, [_I64]),
     "gsr_sort_ws_inst_bytes": (_SZ, [_I64]),
     "gsr_sort_count": (_I, [_P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),

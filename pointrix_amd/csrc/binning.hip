@@ -200,6 +200,27 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// Per digit d (one block each): hist[d][b] <- sum of hist[d][b'] over
+// b' < b (exclusive, in place) and totals[d] <- the digit's total.  Thread t
+// owns a contiguous run of blocks; run sums are scanned across the block.
+__global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restrict__ hist,
+                                                              uint32_t nb,
+                                                              uint32_t *__restrict__ totals) {
+  uint32_t *row = hist + (size_t)blockIdx.x * nb;
+  const uint32_t per = (nb + kThreads - 1) / kThreads;
+  const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += row[b];
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, total);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t v = row[b];
+    row[b] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
 // Stable scatter of one block of kTileItems items.  Each wave ranks its
 // items per digit with ballots (match mask of equal digits, rank = earlier
 // lanes of the mask plus the wave's running count); the block then places
@@ -209,7 +230,8 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
-    uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb) {
+    uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
+    const uint32_t *__restrict__ totals) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
@@ -264,9 +286,11 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     }
     uint32_t total;
     const uint32_t lstart = block_exclusive_scan(run, total);
+    // global start of digit d: all smaller digits, then d in earlier blocks
+    const uint32_t dstart = block_exclusive_scan(d < (1u << bits) ? totals[d] : 0u, total);
 #pragma unroll
     for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][d] += lstart;
-    s_delta[d] = (int32_t)(hist_scanned[d * nb + blockIdx.x] - lstart);
+    s_delta[d] = (int32_t)(dstart + hist_scanned[d * nb + blockIdx.x] - lstart);
   }
   __syncthreads();
 #pragma unroll
@@ -649,14 +673,15 @@ struct SortBufs {
   uint32_t *keyA, *keyB;
   int32_t *valA, *valB;
   uint32_t *hist;      // [256][nb]
-  uint32_t *partials;  // scan partials (max of data scan and hist scan)
+  uint32_t *totals;    // [256] per-digit totals of a pass
+  uint32_t *partials;  // data scan partials
 };
 
 inline size_t sortbufs_bytes(uint64_t cap) {
   const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
   const uint64_t hn = (uint64_t)kRadix * nb;
-  const uint64_t pn = (nblk(cap) > nblk(hn) ? nblk(cap) : nblk(hn)) + 1;
-  return 4 * align256(cap * 4) + align256(hn * 4) + align256(pn * 4);
+  return 4 * align256(cap * 4) + align256(hn * 4) + align256(kRadix * 4) +
+         align256((nb + 1) * 4);
 }
 
 inline char *carve(char *p, size_t bytes, void **out) {
@@ -667,13 +692,13 @@ inline char *carve(char *p, size_t bytes, void **out) {
 inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b) {
   const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
   const uint64_t hn = (uint64_t)kRadix * nb;
-  const uint64_t pn = (nblk(cap) > nblk(hn) ? nblk(cap) : nblk(hn)) + 1;
   p = carve(p, cap * 4, (void **)&b.keyA);
   p = carve(p, cap * 4, (void **)&b.keyB);
   p = carve(p, cap * 4, (void **)&b.valA);
   p = carve(p, cap * 4, (void **)&b.valB);
   p = carve(p, hn * 4, (void **)&b.hist);
-  p = carve(p, pn * 4, (void **)&b.partials);
+  p = carve(p, kRadix * 4, (void **)&b.totals);
+  p = carve(p, (nb + 1) * 4, (void **)&b.partials);
   return p;
 }
 
@@ -694,7 +719,6 @@ inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBit
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
-  const uint32_t hn = kRadix * nb;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
   for (int p = 0; p < passes; ++p) {
@@ -702,9 +726,9 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     const int bits = min(per, nbits - shift);
     const uint32_t mask = (1u << bits) - 1u;
     radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb);
-    run_scan(LoadU32{b.hist}, StoreU32{b.hist}, nullptr, hn, b.partials, nullptr, nullptr, s);
+    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, b.totals);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(k_cur, v_cur, k_alt, v_alt, n_ptr, cap, shift,
-                                                 bits, b.hist, nb);
+                                                 bits, b.hist, nb, b.totals);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;

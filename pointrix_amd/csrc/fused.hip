@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     const float *__restrict__ d_feat, int ld_feat, float *__restrict__ d_xyz, float *__restrict__ d_scale,
     float *__restrict__ d_rot, float *__restrict__ d_shs, float *__restrict__ d_shs_rest,
     float *__restrict__ d_intr, float *__restrict__ d_extr, float *__restrict__ d_center,
-    const float *__restrict__ opac_raw, const float *__restrict__ d_opac_act,
+    const float *__restrict__ opac_raw, const float *__restrict__ d_opac_act, int ld_oact,
     float *__restrict__ d_opac_raw) {
   constexpr int kRest = (KS - 1) * 3;
   __shared__ __align__(16) float s_rest[KS > 1 ? kThreads * kRest : 4];
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       }
       if (d_opac_raw != nullptr) {
         const float a = act_opacity(opac_raw[i]);
-        d_opac_raw[i] = d_opac_act[i] * (a * (1.0f - a));
+        d_opac_raw[i] = d_opac_act[i * ld_oact] * (a * (1.0f - a));
       }
     }
 #pragma unroll
@@ -401,11 +401,12 @@ int launch_bwd_ks(int KE, unsigned nb, hipStream_t st, const float *xyz, const f
                   const float *d_feat, int ld_feat, float *d_xyz,
                   float *d_scale, float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
                   float *d_extr, float *d_center,
-                  const float *opac_raw, const float *d_opac_act, float *d_opac_raw) {
+                  const float *opac_raw, const float *d_opac_act, int ld_oact,
+                  float *d_opac_raw) {
 #define GSR_BWD_ARGS                                                                          \
   xyz, scale, rot, shs, shs_rest, n, intr, extr, center, W, H, nearest, extent, flags, C,      \
       radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, d_rot,   \
-      d_shs, d_shs_rest, d_intr, d_extr, d_center, opac_raw, d_opac_act, d_opac_raw
+      d_shs, d_shs_rest, d_intr, d_extr, d_center, opac_raw, d_opac_act, ld_oact, d_opac_raw
   switch (KE) {
     case 1: preprocess_bwd_kernel<KS, 1><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break;
     case 4: if constexpr (KS >= 4) { preprocess_bwd_kernel<KS, 4><<<nb, kThreads, 0, st>>>(GSR_BWD_ARGS); break; } return GSR_ERR_ARG;
@@ -469,11 +470,12 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
                        const float *d_feat, int ld_feat, float *d_xyz, float *d_scale,
                        float *d_rot, float *d_shs, float *d_shs_rest, float *d_intr,
                        float *d_extr, float *d_center, const float *opacity_raw,
-                       const float *d_opacity_act, float *d_opacity_raw, void *stream) {
+                       const float *d_opacity_act, int ld_opacity_act, float *d_opacity_raw,
+                       void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
                   (flags & ~7) == 0 &&
                   ((opacity_raw == nullptr) == (d_opacity_raw == nullptr)) &&
-                  ((d_opacity_raw == nullptr) || (d_opacity_act != nullptr)) &&
+                  ((d_opacity_raw == nullptr) || (d_opacity_act != nullptr && ld_opacity_act >= 1)) &&
                   (opacity_raw == nullptr || (flags & kFlagRaw)),
               "preprocess_bwd: bad arguments (K=%d flags=%d)", K, flags);
   if (n == 0) return GSR_OK;
@@ -484,7 +486,8 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
 #define GSR_ARGS                                                                               \
   KE, nb, st, xyz, scale, rot, shs, shs_rest, n, intr, extr, cam_center, W, H, nearest, extent, \
       flags, C, radius, d_uv, ld_uv, d_depth, d_conic, ld_conic, d_feat, ld_feat, d_xyz, d_scale, \
-      d_rot, d_shs, d_shs_rest, d_intr, d_extr, d_center, opacity_raw, d_opacity_act, d_opacity_raw
+      d_rot, d_shs, d_shs_rest, d_intr, d_extr, d_center, opacity_raw, d_opacity_act,             \
+      ld_opacity_act, d_opacity_raw
   switch (K) {
     case 1: rc = launch_bwd_ks<1>(GSR_ARGS); break;
     case 4: rc = launch_bwd_ks<4>(GSR_ARGS); break;

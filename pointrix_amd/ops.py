@@ -340,6 +340,7 @@ class _AlphaBlending(torch.autograd.Function):
         ctx.save_for_backward(u, c, o, f, ids_c, tr, T, nc)
         ctx.cfg = (float(bg), int(W), int(H), tile_w, tile_h, opacity.shape)
         ctx.mark_non_differentiable(nc)
+        ctx.set_materialize_grads(False)
         return out, T, nc
 
     @staticmethod
@@ -497,6 +498,7 @@ class _Preprocess(torch.autograd.Function):
                    extr, center, C, K)
         ctx.o_shape = None if opacity_raw is None else opacity_raw.shape
         ctx.mark_non_differentiable(radius, tiles, cov)
+        ctx.set_materialize_grads(False)
         return uv, depth, conic, radius, tiles, feat, cov, o_act
 
     @staticmethod
@@ -518,9 +520,16 @@ class _Preprocess(torch.autograd.Function):
         d_k = z(4) if need_k else None
         d_e = z(12) if need_e else None
         d_c = z(3) if need_c else None
-        d_o = None
+        d_o, ld_o = None, 1
         if o_raw is not None and ctx.needs_input_grad[15]:
-            g_oact = z(n) if g_oact is None else _f32(g_oact).reshape(-1)
+            if g_oact is None:
+                g_oact = z(n)
+            else:
+                # a column of the blend's gradient records is read in place
+                g_oact = g_oact.detach().reshape(-1)
+                if g_oact.dtype != torch.float32 or g_oact.stride(0) < 1:
+                    g_oact = _f32(g_oact)
+                ld_o = g_oact.stride(0)
             d_o = torch.empty_like(o_raw)
         with _span("preprocess_bwd", dev):
             call("gsr_preprocess_bwd", ptr(x), ptr(s), ptr(q), ptr(sh), ptr(rest), n, K, degree,
@@ -528,7 +537,7 @@ class _Preprocess(torch.autograd.Function):
                  g_uv.stride(0), ptr(g_depth), ptr(g_conic), g_conic.stride(0), ptr(g_feat),
                  g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_rest),
                  ptr(d_k), ptr(d_e), ptr(d_c), ptr(o_raw) if d_o is not None else None,
-                 ptr(g_oact) if d_o is not None else None, ptr(d_o), stream_of(x))
+                 ptr(g_oact) if d_o is not None else None, ld_o, ptr(d_o), stream_of(x))
         return (d_x, d_s, d_q, d_sh, d_rest, _intr_grad(d_k, intr) if need_k else None,
                 _extr_grad(d_e, extr) if need_e else None,
                 d_c.reshape(center.shape).to(center.dtype) if need_c else None,

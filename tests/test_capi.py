@@ -47,3 +47,34 @@ def test_struct_layouts_match_bindings():
     assert lib.gsr_adam_entry_bytes() == ctypes.sizeof(optim._Entry)
     assert lib.gsr_plane_grads_bytes() == ctypes.sizeof(ops._PlaneGrads)
     assert lib.gsr_row_copy_bytes() == ctypes.sizeof(densify._RowCopy)
+
+
+def _header_prototypes():
+    src = open(os.path.join(ROOT, "include", "gsr.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"#[^\n]*", "", src)
+    out = {}
+    for m in re.finditer(r"(int|size_t|char\s*\*)\s*(gsr_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", src):
+        out[m.group(2)] = (m.group(1), [p.strip() for p in m.group(3).split(",")])
+    return out
+
+
+def _ctype_of(param):
+    if param in ("void", ""):
+        return None
+    if "*" in param:
+        return ctypes.c_void_p
+    base = param.rsplit(None, 1)[0].replace("const ", "").strip()
+    return {"int64_t": ctypes.c_int64, "int": ctypes.c_int, "float": ctypes.c_float,
+            "double": ctypes.c_double, "size_t": ctypes.c_size_t}[base]
+
+
+def test_binding_argument_types_match_header():
+    """Every ctypes signature has the header's parameter count and types, in
+    order (a missing or shifted argument would pass garbage to a kernel)."""
+    protos = _header_prototypes()
+    assert set(protos) == set(_lib.SIGNATURES)
+    for name, (_, params) in protos.items():
+        want = [t for t in (_ctype_of(p) for p in params) if t is not None]
+        got = list(_lib.SIGNATURES[name][1])
+        assert got == want, (name, [t.__name__ for t in got], [t.__name__ for t in want])
