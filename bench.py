@@ -86,6 +86,20 @@ def l1(a, b):
     return l1_loss(a, b)
 
 
+SETTLE_S = 2.0
+
+
+def settle_steps(step, t_start, min_s=SETTLE_S):
+    """Untimed steps until ``min_s`` seconds have passed since ``t_start``
+    (synchronised every step); returns how many were run."""
+    n = 0
+    while time.perf_counter() - t_start < min_s:
+        step()
+        torch.cuda.synchronize()
+        n += 1
+    return n
+
+
 def make_step(renderer, scene, cam, targets, raw=False):
     """-> (step, loss_fn): ``step(stats)`` = one eager fwd + loss + bwd with
     the gradients dropped afterwards; ``loss_fn()`` = fwd + loss + bwd leaving
@@ -572,16 +586,20 @@ def main():
     step, loss_fn = make_step(renderer, scene, cam, targets)
     print(f"[bench] workload {args.workload}: N={len(scene)} {W}x{H}", file=sys.stderr, flush=True)
 
+    t_w = time.perf_counter()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    # the GPU reaches its sustained clocks only after some hundreds of ms of
+    # load: keep stepping (untimed) until the warmup has lasted SETTLE_S
+    settle = settle_steps(step, t_w)
+    print(f"[bench] warmup done ({args.warmup} steps + {settle} settle steps)", file=sys.stderr,
+          flush=True)
 
-    stats = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(stats if i == args.steps - 1 else None)
+        step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = dt * 1000 / args.steps
@@ -674,6 +692,7 @@ def main():
                    "splats": N, "width": W, "height": H, "visible": Nv, "instances": M,
                    "mpixel_per_s": round(P / 1e6 * 1000.0 / ms, 2),
                    "eager_ms_per_step": round(ms_eager, 4), "graph": graph,
+                   "warmup_settle": {"seconds": SETTLE_S, "extra_untimed_steps": settle},
                    "tight_tiles": tight, "raw_activations": raw,
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
