@@ -306,6 +306,19 @@ int gsr_gather_rows(const GsrRowCopy *table, int ntab, const int32_t *index, int
 int gsr_uv_grad_scale(const float *d_splat, int ld, int64_t n, float sx, float sy, float *out,
                       void *stream);
 
+/* ---- point-cloud activations (gaussian_points.py:30-35,70-81) -----------
+ * scale[N,3] = exp(log_scale), rot[N,4] = quat / max(|quat|, 1e-12),
+ * opacity[N] = sigmoid(logit).  Backward: gradients of the three outputs
+ * (each may be NULL = zero) -> d_log_scale, d_quat, d_logit (each may be NULL
+ * = not wanted), from the forward's inputs and outputs.  quat, rot, g_rot and
+ * d_quat are 16-byte aligned. */
+int gsr_activate_fwd(const float *log_scale, const float *quat, const float *logit, int64_t n,
+                     float *scale, float *rot, float *opacity, void *stream);
+int gsr_activate_bwd(const float *quat, const float *scale, const float *rot,
+                     const float *opacity, const float *g_scale, const float *g_rot,
+                     const float *g_opacity, int64_t n, float *d_log_scale, float *d_quat,
+                     float *d_logit, void *stream);
+
 /* ---- camera pose (camera_model.py:92-175, utils/pose.py:40-84) -----------
  * q[C,4] wxyz (normalised inside as q / max(|q|, 1e-12)), t[C,3] ->
  * E[C,4,4] = [R | t; 0 0 0 1] and center[C,3] = -R^T t (may be NULL).

@@ -66,6 +66,8 @@ SIGNATURES = {
     "gsr_gather_rows": (_I, [_P, _I, _P, _I64, _P]),
     "gsr_densify_stats": (_I, [_P, _I64, _I64, _I, _I64, _P, _P, _F, _F, _P, _P, _P, _P]),
     "gsr_uv_grad_scale": (_I, [_P, _I, _I64, _F, _F, _P, _P]),
+    "gsr_activate_fwd": (_I, [_P, _P, _P, _I64, _P, _P, _P, _P]),
+    "gsr_activate_bwd": (_I, [_P, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "gsr_camera_pose_fwd": (_I, [_P, _P, _I, _P, _P, _P]),
     "gsr_camera_pose_bwd": (_I, [_P, _P, _I, _P, _P, _P, _P, _P]),
     "gsr_plane_grads_bytes": (_I, []),

@@ -659,6 +659,55 @@ def l1_ssim_loss(pred, gt, lambda_ssim=0.2):
 
 
 # ---------------------------------------------------------------------------
+# Point-cloud activations (gaussian_points.py:30-35,70-81)
+# ---------------------------------------------------------------------------
+def _aligned_f32(t):
+    t = _f32(t)
+    return t if t.data_ptr() % 16 == 0 else t.clone()
+
+
+class _Activate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, log_scale, quat, logit):
+        _dev_check(log_scale, quat, logit)
+        ls, q, lo = _f32(log_scale), _aligned_f32(quat), _f32(logit)
+        n = ls.shape[0]
+        if q.shape[0] != n or lo.numel() != n or ls.numel() != 3 * n or q.numel() != 4 * n:
+            raise ValueError("activate: log_scale [N,3], quat [N,4], logit [N,1] expected")
+        s, r, o = torch.empty_like(ls), torch.empty_like(q), torch.empty_like(lo)
+        call("gsr_activate_fwd", ptr(ls), ptr(q), ptr(lo), n, ptr(s), ptr(r), ptr(o),
+             stream_of(ls))
+        ctx.save_for_backward(q, s, r, o)
+        ctx.shapes = (log_scale.shape, quat.shape, logit.shape)
+        ctx.set_materialize_grads(False)
+        return s.view(log_scale.shape), r.view(quat.shape), o.view(logit.shape)
+
+    @staticmethod
+    def backward(ctx, gs, gr, go):
+        q, s, r, o = ctx.saved_tensors
+        n = q.shape[0]
+        need = ctx.needs_input_grad
+        d_ls = torch.empty_like(s) if need[0] else None
+        d_q = torch.empty_like(q) if need[1] else None
+        d_lo = torch.empty_like(o) if need[2] else None
+        gs = None if gs is None else _f32(gs)
+        gr = None if gr is None else _aligned_f32(gr)
+        go = None if go is None else _f32(go)
+        call("gsr_activate_bwd", ptr(q), ptr(s), ptr(r), ptr(o), ptr(gs), ptr(gr), ptr(go), n,
+             ptr(d_ls), ptr(d_q), ptr(d_lo), stream_of(q))
+        shp = ctx.shapes
+        return (None if d_ls is None else d_ls.view(shp[0]),
+                None if d_q is None else d_q.view(shp[1]),
+                None if d_lo is None else d_lo.view(shp[2]))
+
+
+def activate(log_scale, quat, logit):
+    """The point cloud's activations in one kernel (and one backward):
+    (exp(log_scale) [N,3], F.normalize(quat) [N,4], sigmoid(logit) [N,1])."""
+    return _Activate.apply(log_scale, quat, logit)
+
+
+# ---------------------------------------------------------------------------
 # Camera pose (camera_model.py:92-175 with utils/pose.py:40-84)
 # ---------------------------------------------------------------------------
 class _CameraPose(torch.autograd.Function):

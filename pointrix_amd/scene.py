@@ -104,7 +104,8 @@ class GaussianScene:
         return GaussianScene(*ps)
 
     def render_inputs(self, split_sh: bool = False, raw: bool = False) -> dict:
-        """Activated tensors, as BaseModel.forward passes them (base_model.py:66-71).
+        """Activated tensors, as BaseModel.forward passes them (base_model.py:66-71);
+        on the GPU the three activations run as one fused kernel (ops.activate).
 
         split_sh: pass SH as the pair (features, features_rest) instead of
         their concatenation (get_shs, gaussian_points.py:86-90).
@@ -119,6 +120,11 @@ class GaussianScene:
         if raw:
             return {"position": self.position, "opacity": self.opacity,
                     "scaling": self.scaling, "rotation": self.rotation, "shs": shs, "raw": True}
+        if self.scaling.is_cuda:
+            from .ops import activate
+            scaling, rotation, opacity = activate(self.scaling, self.rotation, self.opacity)
+            return {"position": self.position, "opacity": opacity, "scaling": scaling,
+                    "rotation": rotation, "shs": shs}
         return {"position": self.position,
                 "opacity": torch.sigmoid(self.opacity),
                 "scaling": torch.exp(self.scaling),

@@ -102,3 +102,30 @@ def test_output_planes(sizes, nf, used):
     loss.backward()
     loss64.backward()
     assert rel_err(xd.grad, x64.grad) <= 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 1000, 70001])
+def test_activate_matches_torch(n):
+    """ops.activate against the point cloud's torch activations
+    (gaussian_points.py:30-35,70-81), forward within 1e-6 relative and the
+    gradients of each raw parameter within 1e-5 relative; includes one
+    quaternion of zero norm (the clamped branch of F.normalize)."""
+    g = torch.Generator().manual_seed(n)
+    ls = torch.randn(n, 3, generator=g) - 2.0
+    q = torch.randn(n, 4, generator=g) * (0.2 + torch.rand(n, 1, generator=g))
+    lo = torch.randn(n, 1, generator=g) * 2
+    if n > 1:
+        q[n // 2] = 0.0
+    gs, gq, go = (torch.randn(n, 3, generator=g), torch.randn(n, 4, generator=g),
+                  torch.randn(n, 1, generator=g))
+    dev = [t.cuda().requires_grad_(True) for t in (ls, q, lo)]
+    s, r, o = ops.activate(*dev)
+    ((s * gs.cuda()).sum() + (r * gq.cuda()).sum() + (o * go.cuda()).sum()).backward()
+    ref = [t.clone().cuda().requires_grad_(True) for t in (ls, q, lo)]
+    s2, r2, o2 = (torch.exp(ref[0]), torch.nn.functional.normalize(ref[1], dim=-1),
+                  torch.sigmoid(ref[2]))
+    ((s2 * gs.cuda()).sum() + (r2 * gq.cuda()).sum() + (o2 * go.cuda()).sum()).backward()
+    for a, b in ((s, s2), (r, r2), (o, o2)):
+        assert rel_err(a, b) <= 1e-6
+    for a, b in zip(dev, ref):
+        assert rel_err(a.grad, b.grad) <= 1e-5
