@@ -107,6 +107,11 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
   if (d != 0.0f) {
     float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
     float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    // the SH coefficients are requested together with scale and rotation
+    // (one memory round trip instead of two); almost every splat inside the
+    // frustum is also rasterised
+    float sh[KE * 3];
+    load_sh<KS, KE>(shs, shs_rest, i, sh);
     if (flags & kFlagRaw) {
       float qn;
       s = act_scale(s);
@@ -134,8 +139,6 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
       const float3 dir = view_dir(x, cen, len);
       float Y[16];
       sh_basis(dir, KE, Y);
-      float sh[KE * 3];
-      load_sh<KS, KE>(shs, shs_rest, i, sh);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         float acc = sh[c] * Y[0];
@@ -200,8 +203,17 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     float4 q_act = make_float4(1.f, 0.f, 0.f, 0.f);
     float q_norm = 1.0f;
     if (radius[i] > 0) {
+      // every input of the splat is requested up front (one memory round trip)
       float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
       float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+      float sh[KE * 3];
+      load_sh<KS, KE>(shs, shs_rest, i, sh);
+      float gf[7];
+#pragma unroll
+      for (int c = 0; c < 7; ++c) gf[c] = c < C ? d_feat[i * ld_feat + c] : 0.0f;
+      const float guv[2] = {d_uv[i * ld_uv], d_uv[i * ld_uv + 1]};
+      const float g3[3] = {d_conic[i * ld_conic], d_conic[i * ld_conic + 1],
+                           d_conic[i * ld_conic + 2]};
       if (flags & kFlagRaw) {
         s = act_scale(s);
         q = act_rot(q, q_norm);
@@ -214,8 +226,6 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       const float3 dir = view_dir(x, cen, len);
       float Y[16];
       sh_basis(dir, KE, Y);
-      float sh[KE * 3];
-      load_sh<KS, KE>(shs, shs_rest, i, sh);
       float coef[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) coef[k] = 0.0f;
@@ -224,7 +234,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         float acc = sh[c] * Y[0];
 #pragma unroll
         for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
-        const float g = (acc + 0.5f >= 0.0f) ? d_feat[i * ld_feat + c] : 0.0f;
+        const float g = (acc + 0.5f >= 0.0f) ? gf[c] : 0.0f;
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
           dsh[3 * k + c] = g * Y[k];
@@ -245,22 +255,21 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         gcen[1] -= gd.y;
         gcen[2] -= gd.z;
       }
-      int o = 3;
+      // channel layout rgb | depth (flag 1) | normal (flag 2)
+      const bool has_d = (flags & kFlagDepth) != 0;
       float gdepth = d_depth ? d_depth[i] : 0.0f;
-      if (flags & kFlagDepth) gdepth += d_feat[i * ld_feat + (o++)];
+      if (has_d) gdepth += gf[3];
       if (flags & kFlagNormal) {
         NormalFwd nf;
         float nc[3];
         splat_normal(s, q, x, cen, cam, nf, nc);
-        const float gn[3] = {d_feat[i * ld_feat + o], d_feat[i * ld_feat + o + 1],
-                             d_feat[i * ld_feat + o + 2]};
+        const float gn[3] = {has_d ? gf[4] : gf[3], has_d ? gf[5] : gf[4],
+                             has_d ? gf[6] : gf[5]};
         splat_normal_vjp(q, cam, nf, gn, gq, gcam);
       }
-      project_vjp(cam, x, d_uv[i * ld_uv], d_uv[i * ld_uv + 1], gdepth, gx, gcam);
+      project_vjp(cam, x, guv[0], guv[1], gdepth, gx, gcam);
       float cov[6];
       covariance3d(s, q, cov);
-      const float g3[3] = {d_conic[i * ld_conic], d_conic[i * ld_conic + 1],
-                           d_conic[i * ld_conic + 2]};
       float gS[6];
       ewa_vjp(cam, x, cov, W, H, g3, gx, gS, gcam);
       float gs2[3], gq2[4];
