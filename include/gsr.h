@@ -233,6 +233,15 @@ int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opa
                            int tile_h, const float *T_final, const int32_t *n_contrib,
                            const float *d_out, const float *d_T_final, float *d_splat, int ld,
                            void *stream);
+/* As gsr_alpha_blending_bwd, with (d_u, d_v) accumulated into d_uv[N,2] and
+ * d_opacity into d_opacity[N] (dense, caller zero-fills) instead of record
+ * slots 0, 1 and 5, when those pointers are not NULL. */
+int gsr_alpha_blending_bwd_split(const float *uv, const float *conic, const float *opacity,
+                                 const float *feature, int C, const int32_t *ids,
+                                 const int32_t *tile_range, float bg, int W, int H, int tile_w,
+                                 int tile_h, const float *T_final, const int32_t *n_contrib,
+                                 const float *d_out, const float *d_T_final, float *d_splat,
+                                 int ld, float *d_uv, float *d_opacity, void *stream);
 
 /* ---- fused L1 loss (pointrix/model/loss.py:27, mean |x - y|) -----------
  * Graph-safe: two-stage fixed-order reduction into *out (device scalar),

@@ -80,6 +80,8 @@ SIGNATURES = {
                                     _P]),
     "gsr_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,
                                     _P, _P, _I, _P]),
+    "gsr_alpha_blending_bwd_split": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P,
+                                          _P, _P, _P, _P, _I, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

@@ -40,12 +40,12 @@ __global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
     const int32_t *__restrict__ ids, const int32_t *__restrict__ ranges, float bg, int W, int H,
     int gx, const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
     const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
-    int ld) {
+    int ld, float *__restrict__ d_uv, float *__restrict__ d_op) {
   __shared__ BlendLds<C, TW * TH> sm;
   const int tile = blockIdx.x;
   const TileView tv{W, H, gx, tile, 0, (int64_t)H * W, 0};
   blend_bwd_tile<C, TW, TH, TW * TH>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic, opacity,
-                            feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat, ld, sm);
+                            feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat, ld, sm, d_uv, d_op);
 }
 
 template <int C>
@@ -72,12 +72,13 @@ template <int C>
 int launch_bwd(const float *uv, const float *conic, const float *opacity, const float *feature,
                const int32_t *ids, const int32_t *tile_range, float bg, int W, int H, int tw,
                int th, const float *T_final, const int32_t *n_contrib, const float *d_out,
-               const float *d_T_final, float *d_splat, int ld, hipStream_t s) {
+               const float *d_T_final, float *d_splat, int ld, float *d_uv, float *d_op,
+               hipStream_t s) {
   const int gx = (W + tw - 1) / tw, gy = (H + th - 1) / th;
   const unsigned nt = (unsigned)(gx * gy);
 #define GSR_BWD_ARGS                                                                          \
   uv, conic, opacity, feature, ids, tile_range, bg, W, H, gx, T_final, n_contrib, d_out, \
-      d_T_final, d_splat, ld
+      d_T_final, d_splat, ld, d_uv, d_op
   if (tw == 16 && th == 16)
     blend_bwd_kernel<C, 16, 16><<<nt, 256, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 8 && th == 32)
@@ -129,20 +130,31 @@ int gsr_alpha_blending_fwd(const float *uv, const float *conic, const float *opa
   return gsr_launch_status("alpha_blending_fwd");
 }
 
+int gsr_alpha_blending_bwd_split(const float *uv, const float *conic, const float *opacity,
+                                 const float *feature, int C, const int32_t *ids,
+                                 const int32_t *tile_range, float bg, int W, int H, int tile_w,
+                                 int tile_h, const float *T_final, const int32_t *n_contrib,
+                                 const float *d_out, const float *d_T_final, float *d_splat,
+                                 int ld, float *d_uv, float *d_opacity, void *stream) {
+  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16 && ld >= 6 + C,
+              "alpha_blending_bwd: bad sizes (C=%d, ld=%d)", C, ld);
+  int rc = GSR_OK;
+  GSR_DISPATCH_C(C, rc = launch_bwd<CC>(uv, conic, opacity, feature, ids, tile_range, bg, W, H,
+                                        tile_w, tile_h, T_final, n_contrib, d_out, d_T_final,
+                                        d_splat, ld, d_uv, d_opacity, (hipStream_t)stream));
+  GSR_REQUIRE(rc == GSR_OK, "alpha_blending_bwd: unsupported tile shape %dx%d", tile_w, tile_h);
+  return gsr_launch_status("alpha_blending_bwd");
+}
+
 int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opacity,
                            const float *feature, int C, const int32_t *ids,
                            const int32_t *tile_range, float bg, int W, int H, int tile_w,
                            int tile_h, const float *T_final, const int32_t *n_contrib,
                            const float *d_out, const float *d_T_final, float *d_splat, int ld,
                            void *stream) {
-  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16 && ld >= 6 + C,
-              "alpha_blending_bwd: bad sizes (C=%d, ld=%d)", C, ld);
-  int rc = GSR_OK;
-  GSR_DISPATCH_C(C, rc = launch_bwd<CC>(uv, conic, opacity, feature, ids, tile_range, bg, W, H,
-                                        tile_w, tile_h, T_final, n_contrib, d_out, d_T_final,
-                                        d_splat, ld, (hipStream_t)stream));
-  GSR_REQUIRE(rc == GSR_OK, "alpha_blending_bwd: unsupported tile shape %dx%d", tile_w, tile_h);
-  return gsr_launch_status("alpha_blending_bwd");
+  return gsr_alpha_blending_bwd_split(uv, conic, opacity, feature, C, ids, tile_range, bg, W, H,
+                                      tile_w, tile_h, T_final, n_contrib, d_out, d_T_final,
+                                      d_splat, ld, nullptr, nullptr, stream);
 }
 
 }  // extern "C"

@@ -172,7 +172,9 @@ __device__ __forceinline__ void blend_fwd_tile(
 }
 
 // Per-splat gradient record written by the backward: [du, dv, da, db, dc,
-// dopacity, dfeature_0 .. dfeature_{C-1}] at d_splat[g * ld + ...].
+// dopacity, dfeature_0 .. dfeature_{C-1}] at d_splat[g * ld + ...]; (du, dv)
+// and dopacity instead into dense [N,2] / [N] arrays when those are given
+// (their consumers then read 12 bytes per splat, not a 64-byte record row).
 template <int C>
 struct GradLayout {
   static constexpr int Q = 6 + C;
@@ -186,7 +188,8 @@ __device__ __forceinline__ void blend_bwd_tile(
     const float *__restrict__ feat, const int32_t *__restrict__ ids, float bg,
     const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
     const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
-    int ld, BlendLds<C, BD> &sm) {
+    int ld, BlendLds<C, BD> &sm, float *__restrict__ d_uv_sep = nullptr,
+    float *__restrict__ d_op_sep = nullptr) {
   static_assert(BD <= TW * TH, "batch depth exceeds the workgroup");
   constexpr int QP = GradLayout<C>::QP;
   constexpr int Q = GradLayout<C>::Q;
@@ -294,7 +297,14 @@ __device__ __forceinline__ void blend_bwd_tile(
         v[4] = dpow * (-0.5f * dy * dy);
       }
       Bfly<QP, 32>::run(v, lane);
-      if (writer) atomicAdd(d_splat + (int64_t)sm.id[k] * ld + my_q, v[0]);
+      if (writer) {
+        // (du, dv) and dopacity go to their own dense arrays when given
+        const int64_t id = sm.id[k];
+        float *dst = d_splat + id * ld + my_q;
+        if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
+        if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
+        atomicAdd(dst, v[0]);
+      }
     }
   }
 }

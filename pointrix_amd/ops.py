@@ -353,15 +353,19 @@ class _AlphaBlending(torch.autograd.Function):
         g_T = None if g_T is None else _f32(g_T)
         ld = grad_record_width(C)
         rec = torch.zeros(n, ld, device=dev)
+        # (du, dv) and dopacity into dense arrays: their consumers (the uv /
+        # ndc and opacity gradients) then read 12 bytes per splat, not a row
+        d_u = torch.zeros(n, 2, device=dev)
+        d_o = torch.zeros(n, device=dev)
         with _span("alpha_blending_bwd", dev):
-            call("gsr_alpha_blending_bwd", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids), ptr(tr),
-                 bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(rec), ld,
-                 stream_of(u))
-        d_u, d_c, d_o, d_f = rec[:, 0:2], rec[:, 2:5], rec[:, 5], rec[:, 6:6 + C]
+            call("gsr_alpha_blending_bwd_split", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids),
+                 ptr(tr), bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(rec), ld,
+                 ptr(d_u), ptr(d_o), stream_of(u))
+        d_c, d_f = rec[:, 2:5], rec[:, 6:6 + C]
         d_ndc = None
         if ctx.needs_input_grad[9]:
             d_ndc = torch.empty(n, 2, device=dev)
-            call("gsr_uv_grad_scale", ptr(rec), ld, n, 0.5 * W, 0.5 * H, ptr(d_ndc),
+            call("gsr_uv_grad_scale", ptr(d_u), 2, n, 0.5 * W, 0.5 * H, ptr(d_ndc),
                  stream_of(u))
         return (d_u, d_c, d_o.reshape(op_shape), d_f, None, None, None, None, None, d_ndc, None,
                 None)
