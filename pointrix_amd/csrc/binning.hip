@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
-    const uint32_t *__restrict__ totals, int32_t *__restrict__ key_counts) {
+    const uint32_t *__restrict__ totals) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
@@ -307,65 +307,12 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 #pragma unroll 4
   for (int k = 0; k < kItems; ++k) {
     const uint32_t i = k * kThreads + threadIdx.x;
-    const bool valid = i < cnt;
-    const uint32_t kk = valid ? s_key[i] : 0u;
-    if (valid) {
+    if (i < cnt) {
+      const uint32_t kk = s_key[i];
       const uint32_t pos = i + (uint32_t)s_delta[(kk >> shift) & mask];
       if (kout != nullptr) kout[pos] = kk;
       vout[pos] = s_val[i];
     }
-    if (key_counts != nullptr) {
-      // occurrences of each key (the last pass: the block is in final key
-      // order in LDS): per run of equal keys within the wave's 64
-      // consecutive items, its first lane adds the run length to
-      // key_counts[2 * key + 1]
-      const bool start = valid && (lane == 0 || s_key[i - 1] != kk);
-      const uint64_t runs = __ballot(start);
-      const int nvalid = (int)__popcll(__ballot(valid));
-      if (start) {
-        const uint64_t after = lane == 63 ? 0ull : (runs & (~0ull << (lane + 1)));
-        const int next = after != 0ull ? (int)__ffsll((unsigned long long)after) - 1 : 64;
-        atomicAdd(&key_counts[2 * kk + 1], min(next, nvalid) - lane);
-      }
-    }
-  }
-}
-
-// tile_range from per-tile instance counts (left in ranges[2t + 1] by the
-// last sort pass): [start, start + count) with start the exclusive prefix of
-// the counts, (0, 0) for an empty tile -- what scanning the sorted keys for
-// their first and last positions gives.  One 1024-thread block, in order.
-__global__ __launch_bounds__(1024) void ranges_from_counts_kernel(int32_t *ranges, int T) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
-  const int lane = lane_id(), w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < T; base += 1024) {
-    const int t = base + (int)threadIdx.x;
-    const uint32_t c = t < T ? (uint32_t)ranges[2 * t + 1] : 0u;
-    uint32_t inc = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t x = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += x;
-    }
-    if (lane == 63) s_w[w] = inc;
-    __syncthreads();
-    uint32_t wb = 0, tot = 0;
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t x = s_w[k];
-      if (k < w) wb += x;
-      tot += x;
-    }
-    const uint32_t start = s_carry + wb + inc - c;
-    if (t < T) {
-      ranges[2 * t] = c != 0u ? (int32_t)start : 0;
-      ranges[2 * t + 1] = c != 0u ? (int32_t)(start + c) : 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry += tot;
-    __syncthreads();
   }
 }
 
@@ -704,15 +651,39 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(int32_t *__restrict__ p,
   if (i < n) p[i] = v;
 }
 
+// tile_range[t] = [first, last + 1) of tile t in the sorted keys: a key that
+// differs from its predecessor starts a range, one that differs from its
+// successor ends one.  Four consecutive keys per thread (16-byte loads; the
+// key arrays are 256-byte aligned).
 __global__ __launch_bounds__(kThreads) void ranges_kernel(const uint32_t *__restrict__ tkeys,
-                                                          const uint32_t *m_ptr, uint32_t cap,
-                                                          int32_t *__restrict__ ranges) {
-  const uint32_t m = blockIdx.x * kThreads + threadIdx.x;
+                                                           const uint32_t *m_ptr, uint32_t cap,
+                                                           int32_t *__restrict__ ranges) {
+  const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
   const uint32_t M = count_of(m_ptr, cap);
-  if (m >= M) return;
-  const uint32_t t = tkeys[m];
-  if (m == 0 || tkeys[m - 1] != t) ranges[2 * t] = (int32_t)m;
-  if (m == M - 1 || tkeys[m + 1] != t) ranges[2 * t + 1] = (int32_t)(m + 1);
+  const uint32_t m0 = 4 * q;
+  if (m0 >= M) return;
+  uint32_t k[6];  // k[0] = key m0 - 1, k[1..4] = keys m0..m0+3, k[5] = key m0 + 4
+  if (m0 + 4 <= M) {
+    const uint4 v = reinterpret_cast<const uint4 *>(tkeys)[q];
+    k[1] = v.x;
+    k[2] = v.y;
+    k[3] = v.z;
+    k[4] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k[1 + j] = m0 + j < M ? tkeys[m0 + j] : 0u;
+  }
+  k[0] = m0 > 0 ? tkeys[m0 - 1] : ~k[1];
+  k[5] = m0 + 4 < M ? tkeys[m0 + 4] : ~k[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t m = m0 + j;
+    if (m < M) {
+      const uint32_t t = k[1 + j];
+      if (k[j] != t) ranges[2 * t] = (int32_t)m;
+      if (m == M - 1 || k[2 + j] != t) ranges[2 * t + 1] = (int32_t)(m + 1);
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -771,7 +742,7 @@ inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBit
 // is returned through the pointers.
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
-               bool final_keys = true, int32_t *key_counts = nullptr) {
+               bool final_keys = true) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
@@ -784,7 +755,7 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     const bool last = p == passes - 1;
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         k_cur, v_cur, (last && !final_keys) ? nullptr : k_alt, v_alt, n_ptr, cap, shift, bits,
-        b.hist, nb, b.totals, last ? key_counts : nullptr);
+        b.hist, nb, b.totals);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
@@ -890,15 +861,9 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
         v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
-  if (passes == 0) {
-    // one tile: every instance is in it (emission order is depth order)
-    ranges_kernel<<<gsr_blocks(m, kThreads), kThreads, 0, s>>>(kc, g.scalars + 1, (uint32_t)m,
-                                                               tile_range);
-  } else {
-    run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s, false,
-              tile_range);
-    ranges_from_counts_kernel<<<1, 1024, 0, s>>>(tile_range, T);
-  }
+  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
+  ranges_kernel<<<gsr_blocks(((int64_t)m + 3) / 4, kThreads), kThreads, 0, s>>>(
+      kc, g.scalars + 1, (uint32_t)m, tile_range);
   return gsr_launch_status("sort_emit");
 }
 
