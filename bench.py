@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
 # workload, each scaled by the counter's reading for a copy of known size
 # (fetch x2.0, write x1.0); summary in profiles/alpha_blending_bwd_full3m_pmc.md
-PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 98051 * 1024 * 2 + 89086 * 1024,
-                     "alpha_blending_fwd": 75996 * 1024 * 2 + 51523 * 1024,
-                     "preprocess_bwd": 304476 * 1024 * 2 + 680747 * 1024,
-                     "preprocess_fwd": 228052 * 1024 * 2 + 175782 * 1024}
+PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 93594 * 1024 * 2 + 178171 * 1024,
+                     "alpha_blending_fwd": 77478 * 1024 * 2 + 51523 * 1024,
+                     "preprocess_bwd": 306841 * 1024 * 2 + 680692 * 1024,
+                     "preprocess_fwd": 255534 * 1024 * 2 + 175800 * 1024}
 
 
 def rotmat_to_quat(R):
@@ -134,9 +134,9 @@ def make_step(renderer, scene, cam, targets, raw=False):
 def graphed_headline(renderer, scene, cam, loss_fn, steps, warmup, log):
     """The same step captured as one HIP graph (pointrix_amd.graph): timed
     over ``steps`` replays, then three more replays each checked against an
-    eager step (loss, and every gradient within four times the eager
-    run-to-run spread of float-atomic accumulation, at least 1e-5 of the
-    gradient's max)."""
+    eager step (loss, and every gradient within four times the largest eager
+    run-to-run spread of float-atomic accumulation over three eager steps, at
+    least 1e-5 of the gradient's max)."""
     from pointrix_amd.graph import CapturedStep
     leaves = {f"p{i}": p for i, p in enumerate(scene.params() + cam.params())}
     params = list(leaves.values())
@@ -153,7 +153,7 @@ def graphed_headline(renderer, scene, cam, loss_fn, steps, warmup, log):
 
     renderer.workspace.capacity = None
     loss_e, ref = eager()
-    spread = rel(eager()[1], ref)
+    spread = max(rel(eager()[1], ref) for _ in range(3))
     tol = max(1e-5, 4 * spread)
     cs = CapturedStep(loss_fn, leaves, renderer.workspace).capture()
     for _ in range(warmup):
@@ -330,7 +330,8 @@ def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
     log(f"batched {ms_batched:.2f} ms")
     # eager reference for the graph check.  Gradients are accumulated with
     # float atomics, so eager itself varies run to run in the last bits; the
-    # replay tolerance is four times that measured spread (at least 1e-5).
+    # replay tolerance is four times the largest spread measured over three
+    # more eager steps (at least 1e-5).
     def eager_grads():
         loss = loss_b(br.render(cd, **inputs()))
         loss.backward()
@@ -344,7 +345,8 @@ def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
                    for x, y in zip(a, b))
 
     loss_e, ref_grads = eager_grads()
-    noise = grad_rel(eager_grads()[1], ref_grads)
+    # the run-to-run spread of three more eager steps (the largest of them)
+    noise = max(grad_rel(eager_grads()[1], ref_grads) for _ in range(3))
     tol = max(1e-5, 4 * noise)
     gs = GraphedStep(br, cd, scene.named_params(), inputs, loss_b).capture()
     graph_ok = True
