@@ -309,4 +309,164 @@ __device__ __forceinline__ void blend_bwd_tile(
   }
 }
 
+// Backward with two vertically adjacent pixels per lane (TW*TH/2 threads per
+// tile; a wave covers a TW x 128/TW block).  Each lane runs both pixels'
+// reverse recurrences for every splat and adds their contributions before
+// the butterfly, so the cross-lane reduction, the atomic and the LDS reads
+// of the splat are paid once per two pixels.  Same per-pixel decisions and
+// arithmetic as blend_bwd_tile; only the order in which the per-pixel terms
+// are summed differs.
+template <int C>
+struct PixBwd {
+  float T, S, F, last_alpha, T_final, bg_term;
+  float g[C];
+  int last;
+  float fx, fy;
+};
+
+template <int C, int QP>
+__device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, int local,
+                                             const float *f, int fstride, float (&v)[QP]) {
+  const float dx = sp.xy.x - p.fx, dy = sp.xy.y - p.fy;
+  const float power = splat_power(sp.co, dx, dy);
+  const bool near = local < p.last && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
+  const float G = exp_neg(near ? power : 0.0f);
+  const float alpha = fminf(kAlphaMax, sp.co.w * G);
+  const bool contrib = near && !(alpha < kAlphaMin);
+  const float inv_om = __builtin_amdgcn_rcpf(1.0f - alpha);
+  const float Tn = p.T * inv_om;
+  p.T = contrib ? Tn : p.T;
+  const float w = contrib ? alpha * Tn : 0.0f;
+  float fg = f[0] * p.g[0];
+#pragma unroll
+  for (int c = 1; c < C; ++c) fg = fmaf(f[c * fstride], p.g[c], fg);
+#pragma unroll
+  for (int c = 0; c < C; ++c) v[6 + c] = fmaf(w, p.g[c], v[6 + c]);
+  const float S_n = fmaf(p.last_alpha, p.F, (1.0f - p.last_alpha) * p.S);
+  p.S = contrib ? S_n : p.S;
+  p.F = contrib ? fg : p.F;
+  float dL_dalpha = (fg - S_n) * Tn;
+  dL_dalpha -= p.T_final * inv_om * p.bg_term;
+  p.last_alpha = contrib ? alpha : p.last_alpha;
+  // alpha = min(0.99, o G): no gradient where clamped
+  const float gsel = (contrib && sp.co.w * G < kAlphaMax) ? dL_dalpha : 0.0f;
+  v[5] = fmaf(G, gsel, v[5]);
+  const float dpow = alpha * gsel;
+  v[0] = fmaf(dpow, 2.0f * sp.co.x * dx + sp.co.y * dy, v[0]);
+  v[1] = fmaf(dpow, sp.co.y * dx + 2.0f * sp.co.z * dy, v[1]);
+  v[2] = fmaf(dpow, -0.5f * dx * dx, v[2]);
+  v[3] = fmaf(dpow, -dx * dy, v[3]);
+  v[4] = fmaf(dpow, -0.5f * dy * dy, v[4]);
+  return contrib;
+}
+
+template <int C, int TW, int TH, int BD>
+__device__ __forceinline__ void blend_bwd_tile2(
+    const TileView &tv, int start, int end, const float *__restrict__ uv,
+    const float *__restrict__ conic, const float *__restrict__ opacity,
+    const float *__restrict__ feat, const int32_t *__restrict__ ids, float bg,
+    const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
+    const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
+    int ld, BlendLds<C, BD> &sm, float *__restrict__ d_uv_sep, float *__restrict__ d_op_sep) {
+  constexpr int NT = TW * TH / 2;
+  static_assert(BD <= NT, "batch depth exceeds the workgroup");
+  static_assert(TH % 2 == 0 && NT % 64 == 0, "tile shape");
+  constexpr int QP = GradLayout<C>::QP;
+  constexpr int Q = GradLayout<C>::Q;
+  const int tx = tv.tile % tv.gx, ty = tv.tile / tv.gx;
+  const int px = tx * TW + (int)threadIdx.x % TW;
+  const int py0 = ty * TH + 2 * ((int)threadIdx.x / TW);
+  const int lane = lane_id();
+  const int my_q = bfly_index<QP>(lane);
+  const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
+
+  PixBwd<C> pp[2];
+  int tile_last = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    PixBwd<C> &p = pp[h];
+    const int py = py0 + h;
+    const bool inside = px < tv.W && py < tv.H;
+    const int64_t pix = tv.pix_base + (int64_t)py * tv.W + px;
+    p.T = 1.0f;
+    p.last = 0;
+    float gT = 0.0f;
+    if (inside) {
+      p.T = Tfin[pix];
+      p.last = ncon[pix];
+#pragma unroll
+      for (int c = 0; c < C; ++c) p.g[c] = dout[c * tv.chan_stride + pix];
+      if (dTfin != nullptr) gT = dTfin[pix];
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) p.g[c] = 0.0f;
+    }
+    p.T_final = p.T;
+    float gsum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) gsum += p.g[c];
+    p.bg_term = bg * gsum + gT;  // d out / d T_final summed with d T_final
+    p.S = p.F = p.last_alpha = 0.0f;
+    p.fx = (float)px;
+    p.fy = (float)py;
+    tile_last = max(tile_last, p.last);
+  }
+  __syncthreads();  // sm.misc may still be read by the previous tile
+  if (threadIdx.x == 0) sm.misc = 0;
+  __syncthreads();
+  if (tile_last > 0) atomicMax(&sm.misc, tile_last);
+  __syncthreads();
+  const int stop = start + sm.misc;
+  if (stop <= start || end <= start) return;  // uniform
+
+  for (int bend = stop; bend > start; bend -= BD) {
+    const int bstart = max(start, bend - BD);
+    __syncthreads();
+    const int j = bstart + (int)threadIdx.x;
+    if ((int)threadIdx.x < BD && j < bend) {
+      const int g = ids[j];
+      SplatLds sp;
+      sp.xy = make_float2(uv[2 * (int64_t)g], uv[2 * (int64_t)g + 1]);
+      sp.co = make_float4(-0.5f * conic[3 * (int64_t)g], -conic[3 * (int64_t)g + 1],
+                          -0.5f * conic[3 * (int64_t)g + 2], opacity[g - tv.op_offset]);
+      sm.sp[threadIdx.x] = sp;
+      sm.id[threadIdx.x] = g;
+#pragma unroll
+      for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
+    }
+    __syncthreads();
+    for (int k = bend - bstart - 1; k >= 0; --k) {
+      const int local = bstart + k - start;
+      const SplatLds sp = sm.sp[k];
+      // wave-uniform skip when no lane's pixels can use the splat
+      bool any = false;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float dx = sp.xy.x - pp[h].fx, dy = sp.xy.y - pp[h].fy;
+        const float power = splat_power(sp.co, dx, dy);
+        any = any || (local < pp[h].last && !(power > 0.0f) &&
+                      !(power < kPowerCut && sp.co.w <= 1.0f));
+      }
+      if (__ballot(any) == 0ull) continue;
+      float v[QP];
+#pragma unroll
+      for (int q = 0; q < QP; ++q) v[q] = 0.0f;
+      bool contrib = false;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        contrib = pix_bwd_step<C, QP>(pp[h], sp, local, sm.f + k, BD, v) || contrib;
+      if (__ballot(contrib) == 0ull) continue;  // wave-uniform
+      Bfly<QP, 32>::run(v, lane);
+      if (writer) {
+        // (du, dv) and dopacity go to their own dense arrays when given
+        const int64_t id = sm.id[k];
+        float *dst = d_splat + id * ld + my_q;
+        if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
+        if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
+        atomicAdd(dst, v[0]);
+      }
+    }
+  }
+}
+
 }  // namespace gsr
