@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
-    const uint32_t *__restrict__ totals) {
+    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ kgather) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
@@ -310,8 +310,11 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     if (i < cnt) {
       const uint32_t kk = s_key[i];
       const uint32_t pos = i + (uint32_t)s_delta[(kk >> shift) & mask];
-      if (kout != nullptr) kout[pos] = kk;
-      vout[pos] = s_val[i];
+      const int32_t vv = s_val[i];
+      // kgather: the key output receives kgather[value] (a per-item array
+      // carried into the sorted order) instead of the key
+      if (kout != nullptr) kout[pos] = kgather != nullptr ? kgather[vv] : kk;
+      vout[pos] = vv;
     }
   }
 }
@@ -517,11 +520,6 @@ struct StoreCompact {
       vals[excl] = (int32_t)i;
     }
   }
-};
-struct LoadSortedCount {
-  const uint32_t *cnt;
-  const int32_t *ids;
-  __device__ uint32_t operator()(uint32_t s) const { return cnt[ids[s]]; }
 };
 
 // Emission, wave-cooperative: lane l of a wave loads depth-sorted splat
@@ -742,7 +740,7 @@ inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBit
 // is returned through the pointers.
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
-               bool final_keys = true) {
+               bool final_keys = true, const uint32_t *final_gather = nullptr) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
@@ -754,8 +752,8 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, b.totals);
     const bool last = p == passes - 1;
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        k_cur, v_cur, (last && !final_keys) ? nullptr : k_alt, v_alt, n_ptr, cap, shift, bits,
-        b.hist, nb, b.totals);
+        k_cur, v_cur, (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt,
+        n_ptr, cap, shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
@@ -826,13 +824,14 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // compact visible splats with their depth bits as keys -> Nv in scalars[0]
   run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
            g.sb.partials, g.scalars, nullptr, s);
-  // depth sort of the visible splats (4 passes over 32 bits)
+  // depth sort of the visible splats (4 passes over 32 bits); the last pass
+  // writes each splat's instance count, in depth order, where the keys would go
   uint32_t *kc = g.sb.keyA;
   int32_t *vc = g.sb.valA;
-  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false);
+  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt);
   // offsets of each splat's instances, in depth order -> M in scalars[1]
-  run_scan(LoadSortedCount{g.cnt, vc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials,
-           g.scalars + 1, (uint32_t *)m_out, s);
+  run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1,
+           (uint32_t *)m_out, s);
   return gsr_launch_status("sort_count");
 }
 
