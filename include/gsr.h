@@ -154,6 +154,16 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
 int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *radius,
                          const float *conic, const float *opacity, int64_t n, int W, int H,
                          int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, void *stream);
+/* As gsr_sort_count_tight, and M is also copied into m_host (pinned host
+ * memory) as soon as it is known -- before the depth sort, which is queued
+ * behind the copy -- with `event` (a hipEvent_t, may be NULL) recorded after
+ * the copy.  A host that waits on the event reads M while the device still
+ * sorts, instead of draining the stream to read m_out.  No msplat counterpart
+ * (msplat.sort_gaussian, msplat.py:133-135, sizes its outputs internally). */
+int gsr_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
+                         const float *conic, const float *opacity, int64_t n, int W, int H,
+                         int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
+                         int32_t *m_host, void *event, void *stream);
 int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *conic,
                         const float *opacity, int64_t n, int W, int H, int tile_w, int tile_h,
                         void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
@@ -191,6 +201,12 @@ int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius
                       const float *conic, const float *opacity, int64_t n_per_view, int V,
                       const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,
                       int32_t *m_out, void *stream);
+/* As gsr_mv_sort_count, with M also copied into m_host (pinned) ahead of the
+ * depth sort and `event` recorded after the copy (see gsr_sort_count_async). */
+int gsr_mv_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
+                            const float *conic, const float *opacity, int64_t n_per_view, int V,
+                            const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,
+                            int32_t *m_out, int32_t *m_host, void *event, void *stream);
 int gsr_mv_sort_emit(const float *uv, const int32_t *radius, const float *conic,
                      const float *opacity, int64_t n_per_view, int V, const int32_t *view_i,
                      int total_tiles, int tile_w, int tile_h, void *ws_gauss, int64_t m,
@@ -235,7 +251,9 @@ int gsr_alpha_blending_bwd(const float *uv, const float *conic, const float *opa
                            void *stream);
 /* As gsr_alpha_blending_bwd, with (d_u, d_v) accumulated into d_uv[N,2] and
  * d_opacity into d_opacity[N] (dense, caller zero-fills) instead of record
- * slots 0, 1 and 5, when those pointers are not NULL. */
+ * slots 0, 1 and 5, when those pointers are not NULL.  With both given the
+ * records are compact: [d_conic_a, d_conic_b, d_conic_c, d_feature_0 ..
+ * d_feature_{C-1}] (ld >= 3 + C). */
 int gsr_alpha_blending_bwd_split(const float *uv, const float *conic, const float *opacity,
                                  const float *feature, int C, const int32_t *ids,
                                  const int32_t *tile_range, float bg, int W, int H, int tile_w,
@@ -252,6 +270,27 @@ int gsr_l1_loss_fwd(const float *x, const float *y, int64_t n, float *partial, f
                     void *stream);
 int gsr_l1_loss_bwd(const float *x, const float *y, int64_t n, const float *g_out, float *dx,
                     void *stream);
+
+/* ---- weighted sum of L1 terms, sum_k weight[k] * mean |x[k] - y[k]| (the
+ * per-feature L1 terms of a training loss, pointrix/model/loss.py:27 each,
+ * e.g. l1(rgb) + 0.1 l1(depth) + 0.1 l1(normal)).  Same reduction as
+ * gsr_l1_loss_fwd over all terms at once: partial[gsr_l1_sum_partials(terms)]
+ * caller scratch, *out a device scalar; fixed order, no atomics, no memset.
+ * Backward: dx[k] = sign(x[k] - y[k]) * g_out[0] * weight[k] / n[k] for each
+ * term whose dx[k] is not NULL (dx is ignored by the forward). */
+#define GSR_MAX_L1_TERMS 8
+typedef struct {
+  const float *x[GSR_MAX_L1_TERMS];
+  const float *y[GSR_MAX_L1_TERMS];
+  float *dx[GSR_MAX_L1_TERMS];
+  int64_t n[GSR_MAX_L1_TERMS];
+  float weight[GSR_MAX_L1_TERMS];
+  int nterms;
+} GsrL1Terms;
+int gsr_l1_terms_bytes(void); /* sizeof(GsrL1Terms), for bindings */
+int gsr_l1_sum_partials(const GsrL1Terms *terms); /* -1: bad table */
+int gsr_l1_sum_fwd(const GsrL1Terms *terms, float *partial, float *out, void *stream);
+int gsr_l1_sum_bwd(const GsrL1Terms *terms, const float *g_out, void *stream);
 
 /* ---- fused L1 + SSIM (pointrix/model/loss.py:27, :73-116; combined as in
  * base_model.py:112-124) over `planes` planes of W x H ([B,C,H,W]: B*C planes).

@@ -47,9 +47,13 @@ SIGNATURES = {
     "gsr_mv_preprocess_bwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _F, _F, _I, _P,
                                    _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_sort_count_tight": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),
+    "gsr_sort_count_async": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P, _P,
+                                  _P]),
     "gsr_sort_emit_tight": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _I64, _P, _P, _P,
                                  _P]),
     "gsr_mv_sort_count": (_I, [_P, _P, _P, _P, _P, _I64, _I, _P, _I, _I, _P, _P, _P]),
+    "gsr_mv_sort_count_async": (_I, [_P, _P, _P, _P, _P, _I64, _I, _P, _I, _I, _P, _P, _P, _P,
+                                     _P]),
     "gsr_mv_sort_emit": (_I, [_P, _P, _P, _P, _I64, _I, _P, _I, _I, _I, _P, _I64, _P, _P, _P,
                               _P]),
     "gsr_mv_alpha_blending_fwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I64, _I, _P, _I, _I64, _I,
@@ -76,6 +80,10 @@ SIGNATURES = {
     "gsr_l1_loss_partials": (_I, [_I64]),
     "gsr_l1_loss_fwd": (_I, [_P, _P, _I64, _P, _P, _P]),
     "gsr_l1_loss_bwd": (_I, [_P, _P, _I64, _P, _P, _P]),
+    "gsr_l1_terms_bytes": (_I, []),
+    "gsr_l1_sum_partials": (_I, [_P]),
+    "gsr_l1_sum_fwd": (_I, [_P, _P, _P, _P]),
+    "gsr_l1_sum_bwd": (_I, [_P, _P, _P]),
     "gsr_alpha_blending_fwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,
                                     _P]),
     "gsr_alpha_blending_bwd": (_I, [_P, _P, _P, _P, _I, _P, _P, _F, _I, _I, _I, _I, _P, _P, _P,

@@ -17,6 +17,7 @@ larger capacity if the device count outgrew it.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import torch
@@ -155,9 +156,18 @@ class _BatchedFn(torch.autograd.Function):
         g = ws.sort.gauss_buf(nv, dev)
         with ops._span("mv_sort", dev):
             tc, to = (ptr(conic), ptr(o)) if meta["tight"] else (None, None)
-            call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V, ptr(vi),
-                 lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev), st)
-            m = int(m_dev.item()) if capacity is None else int(capacity)
+            if capacity is None:
+                # M reaches the host ahead of the depth sort (no stream drain)
+                m_host, ev = ws.sort.host_count(dev)
+                call("gsr_mv_sort_count_async", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V,
+                     ptr(vi), lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev),
+                     ctypes.c_void_p(m_host.data_ptr()), ctypes.c_void_p(ev.cuda_event), st)
+                ev.synchronize()
+                m = int(m_host[0])
+            else:
+                call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V,
+                     ptr(vi), lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev), st)
+                m = int(capacity)
             if meta["max_instances"] is not None and m > meta["max_instances"]:
                 raise InstanceBudgetExceeded(m, meta["max_instances"])
             ids = torch.empty(max(m, 1), dtype=torch.int32, device=dev)

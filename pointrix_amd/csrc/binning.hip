@@ -82,7 +82,10 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, const ui
 // writes the grand total.
 __global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t *partials, uint32_t nb,
                                                              uint32_t *total_out,
-                                                             uint32_t *total_out2) {
+                                                             uint32_t *total_out2,
+                                                             const uint32_t *__restrict__ aux,
+                                                             uint32_t naux,
+                                                             uint32_t *__restrict__ aux_out) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
   const int lane = lane_id(), w = threadIdx.x >> 6;
@@ -114,6 +117,21 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t *partials,
   if (threadIdx.x == 0) {
     if (total_out) *total_out = s_carry;
     if (total_out2) *total_out2 = s_carry;
+  }
+  if (aux != nullptr) {  // plain sum of aux[0, naux) -> *aux_out
+    uint32_t a = 0;
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < naux; i += 1024) a += aux[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) a += __shfl_xor(a, d, 64);
+    __syncthreads();
+    if (lane == 0) s_w[w] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int k = 0; k < 16; ++k) t += s_w[k];
+      *aux_out = t;
+    }
   }
 }
 
@@ -445,25 +463,37 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
 // Per splat: tile count of its rectangle, and the rectangle packed as
 // (x0 | y0 << 16, w | h << 16) for the emission (which then gathers 8 bytes
 // per splat instead of its position and radius).
+// Instance total of the block's splats -> bsum[blockIdx.x] (their sum is M,
+// known before the depth sort).
+__device__ __forceinline__ void store_block_sum(uint32_t c, uint32_t *__restrict__ bsum) {
+  uint32_t total;
+  block_exclusive_scan(c, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
                                                          const int32_t *__restrict__ radius,
                                                          uint32_t n, Grid g,
                                                          uint32_t *__restrict__ cnt,
-                                                         uint2 *__restrict__ rects) {
+                                                         uint2 *__restrict__ rects,
+                                                         uint32_t *__restrict__ bsum) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const int r = radius[i];
   uint32_t c = 0;
-  uint2 packed = make_uint2(0u, 0u);
-  if (r > 0) {
-    const ItemGrid ig = item_grid(g, i);
-    const Rect q = tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
-    const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
-    c = w * h;
-    packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
+  if (i < n) {
+    const int r = radius[i];
+    uint2 packed = make_uint2(0u, 0u);
+    if (r > 0) {
+      const ItemGrid ig = item_grid(g, i);
+      const Rect q =
+          tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
+      const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
+      c = w * h;
+      packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
+    }
+    cnt[i] = c;
+    rects[i] = packed;
   }
-  cnt[i] = c;
-  rects[i] = packed;
+  store_block_sum(c, bsum);
 }
 
 // Tight counts: per splat the number of rectangle tiles tile_reachable
@@ -473,37 +503,40 @@ __global__ __launch_bounds__(kThreads) void count_tight_kernel(const float *__re
                                                                const int32_t *__restrict__ radius,
                                                                uint32_t n, Grid g,
                                                                uint32_t *__restrict__ cnt,
-                                                               uint64_t *__restrict__ masks) {
+                                                               uint64_t *__restrict__ masks,
+                                                               uint32_t *__restrict__ bsum) {
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
-  if (wave_base >= n) return;  // uniform per wave
-  const int lane = lane_id();
-  const uint32_t i = wave_base + lane;
-  const TLane L = tight_lane(g, uv, radius, i, i < n);
-  int total;
-  const int P = wave_excl_scan(L.area, total);
   uint32_t mine = 0;
-  uint64_t mbits = 0;
-  for (int c0 = 0; c0 < total; c0 += 64) {
-    const uint32_t c = (uint32_t)(c0 + lane);
-    const int o = tight_owner(P, c);
-    const int Po = shfl_i(P, o), w = shfl_i(L.w, o), x0 = shfl_i(L.x0, o), y0 = shfl_i(L.y0, o);
-    const float inv_w = shfl_f(L.inv_w, o);
-    int tx, ty;
-    cand_tile((int)c - Po, w, inv_w, x0, y0, tx, ty);
-    const bool test = owner_test(L, o, tx, ty, g.tw, g.th);
-    const uint64_t bal = __ballot(c < (uint32_t)total && test);
-    const int lo = max(P, c0) - c0, hi = min(P + L.area, c0 + 64) - c0;
-    if (hi > lo) {
-      const uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
-      const uint64_t piece = (bal & up) >> lo;
-      mine += (uint32_t)__popcll(piece);
-      mbits |= piece << (max(P, c0) - P);  // only kept for areas <= 64
+  if (wave_base < n) {  // uniform per wave
+    const int lane = lane_id();
+    const uint32_t i = wave_base + lane;
+    const TLane L = tight_lane(g, uv, radius, i, i < n);
+    int total;
+    const int P = wave_excl_scan(L.area, total);
+    uint64_t mbits = 0;
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      const uint32_t c = (uint32_t)(c0 + lane);
+      const int o = tight_owner(P, c);
+      const int Po = shfl_i(P, o), w = shfl_i(L.w, o), x0 = shfl_i(L.x0, o), y0 = shfl_i(L.y0, o);
+      const float inv_w = shfl_f(L.inv_w, o);
+      int tx, ty;
+      cand_tile((int)c - Po, w, inv_w, x0, y0, tx, ty);
+      const bool test = owner_test(L, o, tx, ty, g.tw, g.th);
+      const uint64_t bal = __ballot(c < (uint32_t)total && test);
+      const int lo = max(P, c0) - c0, hi = min(P + L.area, c0 + 64) - c0;
+      if (hi > lo) {
+        const uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+        const uint64_t piece = (bal & up) >> lo;
+        mine += (uint32_t)__popcll(piece);
+        mbits |= piece << (max(P, c0) - P);  // only kept for areas <= 64
+      }
+    }
+    if (i < n) {
+      cnt[i] = mine;
+      if (L.area > 0 && L.area <= 64) masks[i] = mbits;
     }
   }
-  if (i < n) {
-    cnt[i] = mine;
-    if (L.area > 0 && L.area <= 64) masks[i] = mbits;
-  }
+  store_block_sum(mine, bsum);
 }
 
 struct LoadFlag {
@@ -726,10 +759,12 @@ inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b) {
 
 template <class Load, class Store>
 void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap, uint32_t *partials,
-              uint32_t *total_out, uint32_t *total_out2, hipStream_t s) {
+              uint32_t *total_out, uint32_t *total_out2, hipStream_t s,
+              const uint32_t *aux = nullptr, uint32_t naux = 0, uint32_t *aux_out = nullptr) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   scan_reduce_kernel<<<nb, kThreads, 0, s>>>(ld, n_ptr, cap, partials);
-  scan_partials_kernel<<<1, 1024, 0, s>>>(partials, nb, total_out, total_out2);
+  scan_partials_kernel<<<1, 1024, 0, s>>>(partials, nb, total_out, total_out2, aux, naux,
+                                          aux_out);
   scan_down_kernel<<<nb, kThreads, 0, s>>>(ld, st, n_ptr, cap, partials);
 }
 
@@ -772,6 +807,7 @@ int tile_bits(int T) {
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
   uint32_t *cnt, *offs, *scalars;  // scalars[0] = Nv, scalars[1] = M
+  uint32_t *bsum;                  // instances per count block
   uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
   uint2 *rects;                    // exact binning: packed tile rectangles
   SortBufs sb;
@@ -779,7 +815,8 @@ struct GaussWs {
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
-  return 2 * align256(c * 4) + align256(64) + 2 * align256(c * 8) + sortbufs_bytes(c);
+  return 2 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
+         2 * align256(c * 8) + sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -788,6 +825,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 4, (void **)&g.cnt);
   p = carve(p, c * 4, (void **)&g.offs);
   p = carve(p, 64, (void **)&g.scalars);
+  p = carve(p, (c / kThreads + 1) * 4, (void **)&g.bsum);
   p = carve(p, c * 8, (void **)&g.masks);
   p = carve(p, c * 8, (void **)&g.rects);
   carve_sortbufs(p, c, g.sb);
@@ -805,33 +843,53 @@ size_t gsr_sort_ws_inst_bytes(int64_t m) { return sortbufs_bytes((uint64_t)(m > 
 
 namespace {
 
+// M to the host copy as soon as it is known (m_host, pinned; event recorded
+// after the copy), ahead of the depth sort that is queued behind it.
+int copy_count_to_host(const int32_t *m_out, int32_t *m_host, void *event, hipStream_t s) {
+  if (m_host == nullptr) return GSR_OK;
+  GSR_REQUIRE(hipMemcpyAsync(m_host, m_out, sizeof(int32_t), hipMemcpyDeviceToHost, s) ==
+                  hipSuccess,
+              "sort_count: instance count copy failed");
+  if (event != nullptr)
+    GSR_REQUIRE(hipEventRecord((hipEvent_t)event, s) == hipSuccess,
+                "sort_count: event record failed");
+  return GSR_OK;
+}
+
 int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, int64_t n,
-                    const Grid &gr, void *ws_gauss, int32_t *m_out, hipStream_t s) {
+                    const Grid &gr, void *ws_gauss, int32_t *m_out, hipStream_t s,
+                    int32_t *m_host = nullptr, void *event = nullptr) {
   GaussWs g;
   carve_gauss(ws_gauss, (uint64_t)n, g);
   const uint32_t cap = (uint32_t)(n > 0 ? n : 1);
   if (n == 0) {
     fill_kernel<<<1, kThreads, 0, s>>>((int32_t *)g.scalars, 2, 0);
     fill_kernel<<<1, kThreads, 0, s>>>(m_out, 1, 0);
-    return gsr_launch_status("sort_count");
+    const int rc = copy_count_to_host(m_out, m_host, event, s);
+    return rc != GSR_OK ? rc : gsr_launch_status("sort_count");
   }
+  const uint32_t ncb = (uint32_t)gsr_blocks(n, kThreads);
   if (gr.conic == nullptr)
-    count_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt,
-                                                              g.rects);
+    count_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.rects, g.bsum);
   else
-    count_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr,
-                                                                    g.cnt, g.masks);
-  // compact visible splats with their depth bits as keys -> Nv in scalars[0]
+    count_tight_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.masks,
+                                                g.bsum);
+  // compact visible splats with their depth bits as keys -> Nv in scalars[0];
+  // the same launch sums the count blocks' totals -> M in m_out
   run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
-           g.sb.partials, g.scalars, nullptr, s);
+           g.sb.partials, g.scalars, nullptr, s, g.bsum, ncb, (uint32_t *)m_out);
+  {
+    const int rc = copy_count_to_host(m_out, m_host, event, s);
+    if (rc != GSR_OK) return rc;
+  }
   // depth sort of the visible splats (4 passes over 32 bits); the last pass
   // writes each splat's instance count, in depth order, where the keys would go
   uint32_t *kc = g.sb.keyA;
   int32_t *vc = g.sb.valA;
   run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt);
   // offsets of each splat's instances, in depth order -> M in scalars[1]
-  run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1,
-           (uint32_t *)m_out, s);
+  run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
+           s);
   return gsr_launch_status("sort_count");
 }
 
@@ -882,6 +940,21 @@ int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *rad
   return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
 }
 
+int gsr_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
+                         const float *conic, const float *opacity, int64_t n, int W, int H,
+                         int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
+                         int32_t *m_host, void *event, void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0 &&
+                  (W + tile_w - 1) / tile_w < 65536 && (H + tile_h - 1) / tile_h < 65536,
+              "sort_count: bad sizes");
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "sort_count: conic without opacity");
+  GSR_REQUIRE(m_host != nullptr, "sort_count_async: no host count buffer");
+  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0,
+                W, H, conic, opacity};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream, m_host,
+                         event);
+}
+
 int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *conic,
                         const float *opacity, int64_t n, int W, int H, int tile_w, int tile_h,
                         void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
@@ -921,6 +994,21 @@ int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius
   const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1),
                 0, 0, conic, opacity};
   return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream);
+}
+
+int gsr_mv_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
+                            const float *conic, const float *opacity, int64_t n_per_view, int V,
+                            const int32_t *view_int, int tile_w, int tile_h, void *ws_gauss,
+                            int32_t *m_out, int32_t *m_host, void *event, void *stream) {
+  const int64_t n = n_per_view * (int64_t)V;
+  GSR_REQUIRE(V >= 1 && n_per_view >= 0 && n < (int64_t)INT32_MAX && tile_w > 0 && tile_h > 0,
+              "mv_sort_count: bad sizes");
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "mv_sort_count: conic without opacity");
+  GSR_REQUIRE(m_host != nullptr, "mv_sort_count_async: no host count buffer");
+  const Grid gr{0, 0, tile_w, tile_h, view_int, (uint32_t)(n_per_view > 0 ? n_per_view : 1),
+                0, 0, conic, opacity};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream, m_host,
+                         event);
 }
 
 int gsr_mv_sort_emit(const float *uv, const int32_t *radius, const float *conic,

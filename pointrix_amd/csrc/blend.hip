@@ -153,7 +153,8 @@ int gsr_alpha_blending_bwd_split(const float *uv, const float *conic, const floa
                                  int tile_h, const float *T_final, const int32_t *n_contrib,
                                  const float *d_out, const float *d_T_final, float *d_splat,
                                  int ld, float *d_uv, float *d_opacity, void *stream) {
-  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16 && ld >= 6 + C,
+  const bool compact = d_uv != nullptr && d_opacity != nullptr;
+  GSR_REQUIRE(W > 0 && H > 0 && C >= 1 && C <= 16 && ld >= (compact ? 3 : 6) + C,
               "alpha_blending_bwd: bad sizes (C=%d, ld=%d)", C, ld);
   int rc = GSR_OK;
   GSR_DISPATCH_C(C, rc = launch_bwd<CC>(uv, conic, opacity, feature, ids, tile_range, bg, W, H,

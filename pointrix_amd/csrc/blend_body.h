@@ -174,7 +174,8 @@ __device__ __forceinline__ void blend_fwd_tile(
 // Per-splat gradient record written by the backward: [du, dv, da, db, dc,
 // dopacity, dfeature_0 .. dfeature_{C-1}] at d_splat[g * ld + ...]; (du, dv)
 // and dopacity instead into dense [N,2] / [N] arrays when those are given
-// (their consumers then read 12 bytes per splat, not a 64-byte record row).
+// (their consumers then read 12 bytes per splat, not a record row), and with
+// both given the record row is compact, [da, db, dc, dfeature...] (3 + C).
 template <int C>
 struct GradLayout {
   static constexpr int Q = 6 + C;
@@ -202,6 +203,11 @@ __device__ __forceinline__ void blend_bwd_tile(
   const int lane = lane_id();
   const int my_q = bfly_index<QP>(lane);
   const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
+  // record column of this lane's quantity: compact rows [da, db, dc,
+  // dfeature...] when both dense arrays are given, else the full row
+  const int rec_col = (d_uv_sep != nullptr && d_op_sep != nullptr)
+                          ? my_q - (my_q < 5 ? 2 : 3)
+                          : my_q;
 
   float T = 1.0f, gpix[C];
   int last = 0;
@@ -300,7 +306,7 @@ __device__ __forceinline__ void blend_bwd_tile(
       if (writer) {
         // (du, dv) and dopacity go to their own dense arrays when given
         const int64_t id = sm.id[k];
-        float *dst = d_splat + id * ld + my_q;
+        float *dst = d_splat + id * ld + rec_col;
         if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
         if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
         atomicAdd(dst, v[0]);
@@ -379,6 +385,11 @@ __device__ __forceinline__ void blend_bwd_tile2(
   const int lane = lane_id();
   const int my_q = bfly_index<QP>(lane);
   const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
+  // record column of this lane's quantity: compact rows [da, db, dc,
+  // dfeature...] when both dense arrays are given, else the full row
+  const int rec_col = (d_uv_sep != nullptr && d_op_sep != nullptr)
+                          ? my_q - (my_q < 5 ? 2 : 3)
+                          : my_q;
 
   PixBwd<C> pp[2];
   int tile_last = 0;
@@ -460,7 +471,7 @@ __device__ __forceinline__ void blend_bwd_tile2(
       if (writer) {
         // (du, dv) and dopacity go to their own dense arrays when given
         const int64_t id = sm.id[k];
-        float *dst = d_splat + id * ld + my_q;
+        float *dst = d_splat + id * ld + rec_col;
         if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
         if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
         atomicAdd(dst, v[0]);

@@ -255,6 +255,17 @@ class SortWorkspace:
             self.m_dev = torch.zeros(1, dtype=torch.int32, device=device)
         return self.m_dev
 
+    def host_count(self, device):
+        """(pinned host int32[1], event) the instance count is copied into
+        ahead of the depth sort; the event is created here (a first record)
+        so its handle exists before the library records it."""
+        if getattr(self, "m_host", None) is None or self.m_host_dev != device:
+            self.m_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            self.m_event = torch.cuda.Event()
+            self.m_event.record(torch.cuda.current_stream(device))
+            self.m_host_dev = device
+        return self.m_host, self.m_event
+
     def instances(self) -> int:
         """Device instance count of the last sort (host read)."""
         return int(self.m_dev.item()) if self.m_dev is not None else 0
@@ -298,14 +309,22 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
         conic = _f32(conic)
         opacity = _f32(opacity).reshape(-1)
     tc, to = (None, None) if conic is None else (ptr(conic), ptr(opacity))
-    with _span("sort_count", dev):
-        call("gsr_sort_count_tight", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H), tile_w,
-             tile_h, ptr(g), ptr(m_dev), st)
     if ws.capacity is None:
-        m = int(m_dev.item())
+        # M reaches the host ahead of the depth sort: waiting for it does not
+        # drain the stream, so the sort keeps the device busy meanwhile
+        m_host, ev = ws.host_count(dev)
+        with _span("sort_count", dev):
+            call("gsr_sort_count_async", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H),
+                 tile_w, tile_h, ptr(g), ptr(m_dev), ctypes.c_void_p(m_host.data_ptr()),
+                 ctypes.c_void_p(ev.cuda_event), st)
+        ev.synchronize()
+        m = int(m_host[0])
         global last_instance_count
         last_instance_count = m
     else:
+        with _span("sort_count", dev):
+            call("gsr_sort_count_tight", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H),
+                 tile_w, tile_h, ptr(g), ptr(m_dev), st)
         m = int(ws.capacity)
     ids = torch.empty(m, dtype=torch.int32, device=dev)
     tile_range = torch.empty(gx * gy, 2, dtype=torch.int32, device=dev)
@@ -351,17 +370,23 @@ class _AlphaBlending(torch.autograd.Function):
         dev = u.device
         g_out = torch.zeros(C, H, W, device=dev) if g_out is None else _f32(g_out)
         g_T = None if g_T is None else _f32(g_T)
-        ld = grad_record_width(C)
-        rec = torch.zeros(n, ld, device=dev)
-        # (du, dv) and dopacity into dense arrays: their consumers (the uv /
-        # ndc and opacity gradients) then read 12 bytes per splat, not a row
-        d_u = torch.zeros(n, 2, device=dev)
-        d_o = torch.zeros(n, device=dev)
+        # one zero-filled buffer: compact records [da, db, dc, dfeature...]
+        # (3 + C per splat), then (du, dv) [n,2] and dopacity [n] dense, so
+        # their consumers (the uv / ndc and opacity gradients) read 12 bytes
+        # per splat, not a row
+        ld = 3 + C
+        a64 = lambda v: (v + 63) // 64 * 64  # 256-byte aligned sections
+        off_u = a64(n * ld)
+        off_o = off_u + a64(2 * n)
+        buf = torch.zeros(off_o + n, device=dev)
+        rec = buf[:n * ld].view(n, ld)
+        d_u = buf[off_u:off_u + 2 * n].view(n, 2)
+        d_o = buf[off_o:off_o + n]
         with _span("alpha_blending_bwd", dev):
             call("gsr_alpha_blending_bwd_split", ptr(u), ptr(c), ptr(o), ptr(f), C, ptr(ids),
                  ptr(tr), bg, W, H, tw, th, ptr(T), ptr(nc), ptr(g_out), ptr(g_T), ptr(rec), ld,
                  ptr(d_u), ptr(d_o), stream_of(u))
-        d_c, d_f = rec[:, 2:5], rec[:, 6:6 + C]
+        d_c, d_f = rec[:, 0:3], rec[:, 3:3 + C]
         d_ndc = None
         if ctx.needs_input_grad[9]:
             d_ndc = torch.empty(n, 2, device=dev)
@@ -521,9 +546,13 @@ class _Preprocess(torch.autograd.Function):
         d_rest = None if rest is None else torch.empty_like(rest)
         need_k, need_e, need_c = ctx.needs_input_grad[5], ctx.needs_input_grad[6], \
             ctx.needs_input_grad[7]
-        d_k = z(4) if need_k else None
-        d_e = z(12) if need_e else None
-        d_c = z(3) if need_c else None
+        # the camera gradients share one zeroed buffer (one fill): intrinsics
+        # [0:4], extrinsics [4:20] as a 4x4 whose bottom row stays zero, centre
+        # [20:23]
+        cam = z(24) if (need_k or need_e or need_c) else None
+        d_k = cam[0:4] if need_k else None
+        d_e = cam[4:16] if need_e else None
+        d_c = cam[20:23] if need_c else None
         d_o, ld_o = None, 1
         if o_raw is not None and ctx.needs_input_grad[15]:
             if g_oact is None:
@@ -542,8 +571,14 @@ class _Preprocess(torch.autograd.Function):
                  g_feat.stride(0), ptr(d_x), ptr(d_s), ptr(d_q), ptr(d_sh), ptr(d_rest),
                  ptr(d_k), ptr(d_e), ptr(d_c), ptr(o_raw) if d_o is not None else None,
                  ptr(g_oact) if d_o is not None else None, ld_o, ptr(d_o), stream_of(x))
-        return (d_x, d_s, d_q, d_sh, d_rest, _intr_grad(d_k, intr) if need_k else None,
-                _extr_grad(d_e, extr) if need_e else None,
+        g_k = g_e = None
+        if need_k:
+            g_k = (cam[0:4].view(intr.shape).to(intr.dtype) if intr.numel() == 4
+                   else _intr_grad(d_k, intr))
+        if need_e:
+            g_e = (cam[4:20].view(4, 4).to(extr.dtype) if tuple(extr.shape) == (4, 4)
+                   else _extr_grad(d_e, extr))
+        return (d_x, d_s, d_q, d_sh, d_rest, g_k, g_e,
                 d_c.reshape(center.shape).to(center.dtype) if need_c else None,
                 None, None, None, None, None, None, None,
                 None if d_o is None else d_o.view(ctx.o_shape))
@@ -614,6 +649,73 @@ class _L1Loss(torch.autograd.Function):
 def l1_loss(x, y):
     """mean |x - y| as one deterministic, graph-capturable reduction."""
     return _L1Loss.apply(x, y)
+
+
+class _L1Terms(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p * 8), ("y", ctypes.c_void_p * 8),
+                ("dx", ctypes.c_void_p * 8), ("n", ctypes.c_int64 * 8),
+                ("weight", ctypes.c_float * 8), ("nterms", ctypes.c_int)]
+
+
+def _l1_table(xs, ys, weights, dxs=None):
+    tab = _L1Terms()
+    for k, (x, y, w) in enumerate(zip(xs, ys, weights)):
+        tab.x[k], tab.y[k] = x.data_ptr(), y.data_ptr()
+        tab.dx[k] = None if dxs is None or dxs[k] is None else dxs[k].data_ptr()
+        tab.n[k], tab.weight[k] = x.numel(), w
+    tab.nterms = len(xs)
+    return tab
+
+
+class _L1Sum(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, weights, *xy):
+        xs, ys = [_f32(t) for t in xy[0::2]], [_f32(t) for t in xy[1::2]]
+        _dev_check(*xs, *ys)
+        tab = _l1_table(xs, ys, weights)
+        lib = _lib.load()
+        nb = int(lib.gsr_l1_sum_partials(ctypes.byref(tab)))
+        if nb <= 0:
+            raise ValueError("l1_loss_sum: every term must be non-empty")
+        partial = torch.empty(nb, device=xs[0].device)
+        out = torch.empty((), device=xs[0].device)
+        call("gsr_l1_sum_fwd", ctypes.byref(tab), ptr(partial), ptr(out), stream_of(xs[0]))
+        ctx.save_for_backward(*xs, *ys)
+        ctx.weights = weights
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        saved = ctx.saved_tensors
+        k = len(saved) // 2
+        xs, ys = saved[:k], saved[k:]
+        need = [ctx.needs_input_grad[1 + 2 * i] or ctx.needs_input_grad[2 + 2 * i]
+                for i in range(k)]
+        dxs = [torch.empty_like(x) if nd else None for x, nd in zip(xs, need)]
+        call("gsr_l1_sum_bwd", ctypes.byref(_l1_table(xs, ys, ctx.weights, dxs)),
+             ptr(_f32(g)), stream_of(xs[0]))
+        grads = [None]
+        for i in range(k):
+            grads.append(dxs[i] if ctx.needs_input_grad[1 + 2 * i] else None)
+            grads.append(-dxs[i] if ctx.needs_input_grad[2 + 2 * i] else None)
+        return tuple(grads)
+
+
+def l1_loss_sum(terms):
+    """sum of w * mean |x - y| over ``terms`` = [(x, y, w), ...] (at most 8),
+    e.g. l1(rgb) + 0.1 l1(depth) + 0.1 l1(normal) as one two-kernel reduction
+    and a one-kernel backward, instead of a reduction per term plus the
+    scalar multiply / add kernels (and their backward) that combine them."""
+    terms = list(terms)
+    if not 1 <= len(terms) <= 8:
+        raise ValueError("l1_loss_sum takes 1..8 terms")
+    flat, weights = [], []
+    for x, y, w in terms:
+        if x.shape != y.shape:
+            raise ValueError(f"l1_loss_sum shapes differ: {tuple(x.shape)} vs {tuple(y.shape)}")
+        flat += [x, y]
+        weights.append(float(w))
+    return _L1Sum.apply(tuple(weights), *flat)
 
 
 # ---------------------------------------------------------------------------

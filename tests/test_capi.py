@@ -47,6 +47,7 @@ def test_struct_layouts_match_bindings():
     assert lib.gsr_adam_entry_bytes() == ctypes.sizeof(optim._Entry)
     assert lib.gsr_plane_grads_bytes() == ctypes.sizeof(ops._PlaneGrads)
     assert lib.gsr_row_copy_bytes() == ctypes.sizeof(densify._RowCopy)
+    assert lib.gsr_l1_terms_bytes() == ctypes.sizeof(ops._L1Terms)
 
 
 def _header_prototypes():

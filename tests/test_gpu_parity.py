@@ -273,3 +273,36 @@ def test_l1_loss(device, n):
     ref.backward()
     assert abs(loss.item() - ref.item()) <= 1e-6 * max(1.0, abs(ref.item()))
     assert torch.allclose(x.grad.double().cpu(), x64.grad, rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("sizes", [[1], [3, 1, 3], [5, 2, 1, 7, 3, 1, 2, 4]])
+@pytest.mark.parametrize("P", [17, 1237 * 822])
+def test_l1_loss_sum(device, sizes, P):
+    """sum_k w_k mean|x_k - y_k| against the FP64 sum of separate means; the
+    x_k are planes of one image split along dim 0 (as the renderer's output
+    planes are), one target also takes a gradient and one term's input
+    none."""
+    g = torch.Generator().manual_seed(len(sizes) * 7 + P)
+    img = torch.rand(sum(sizes), P, generator=g)
+    tg = [torch.rand(c, P, generator=g) for c in sizes]
+    w = [1.0 if k == 0 else 0.1 * (k + 1) for k in range(len(sizes))]
+    xd = img.to(device).requires_grad_(True)
+    xs = list(torch.split(xd, sizes, dim=0))
+    ys = [t.to(device) for t in tg]
+    ys[-1].requires_grad_(True)
+    frozen = len(sizes) > 2
+    if frozen:
+        xs[1] = xs[1].detach()
+    loss = ops.l1_loss_sum([(x, y, wk) for x, y, wk in zip(xs, ys, w)])
+    loss.backward()
+    x64 = img.double().requires_grad_(True)
+    y64 = tg[-1].double().requires_grad_(True)
+    xs64 = list(torch.split(x64, sizes, dim=0))
+    if frozen:
+        xs64[1] = xs64[1].detach()
+    ys64 = [t.double() for t in tg[:-1]] + [y64]
+    ref = sum(wk * (x - y).abs().mean() for x, y, wk in zip(xs64, ys64, w))
+    ref.backward()
+    assert abs(loss.item() - ref.item()) <= 1e-6 * max(1.0, abs(ref.item()))
+    assert torch.allclose(xd.grad.double().cpu(), x64.grad, rtol=1e-6, atol=1e-12)
+    assert torch.allclose(ys[-1].grad.double().cpu(), y64.grad, rtol=1e-6, atol=1e-12)
