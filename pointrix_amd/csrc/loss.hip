@@ -13,6 +13,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 1024;
+constexpr int kBwdItems = 4 * kThreads;  // elements per block of the multi-term backward
 
 __global__ __launch_bounds__(kThreads) void l1_partial_kernel(const float *__restrict__ x,
                                                               const float *__restrict__ y,
@@ -89,10 +90,15 @@ __global__ __launch_bounds__(kThreads) void l1_sum_partial_kernel(L1Table t,
   const float *__restrict__ x = t.x[k];
   const float *__restrict__ y = t.y[k];
   const int64_t n = t.n[k];
+  const int64_t stride = (int64_t)nbk * kThreads;
+  int64_t i = (int64_t)(blockIdx.x - t.first_block[k]) * kThreads + threadIdx.x;
   float acc = 0.0f;
-  for (int64_t i = (int64_t)(blockIdx.x - t.first_block[k]) * kThreads + threadIdx.x; i < n;
-       i += (int64_t)nbk * kThreads)
-    acc += fabsf(x[i] - y[i]);
+  for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
+    const float a0 = x[i], a1 = x[i + stride], a2 = x[i + 2 * stride], a3 = x[i + 3 * stride];
+    const float b0 = y[i], b1 = y[i + stride], b2 = y[i + 2 * stride], b3 = y[i + 3 * stride];
+    acc += (fabsf(a0 - b0) + fabsf(a1 - b1)) + (fabsf(a2 - b2) + fabsf(a3 - b3));
+  }
+  for (; i < n; i += stride) acc += fabsf(x[i] - y[i]);
   acc = wave_sum(acc);
   if (lane_id() == 0) s_w[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -108,11 +114,16 @@ __global__ __launch_bounds__(kThreads) void l1_sum_bwd_kernel(L1Table t,
   const int k = term_of_block(t, blockIdx.x);
   float *__restrict__ dx = t.dx[k];
   if (dx == nullptr) return;
-  const int64_t i = (int64_t)(blockIdx.x - t.first_block[k]) * kThreads + threadIdx.x;
-  if (i >= t.n[k]) return;
-  const float d = t.x[k][i] - t.y[k][i];
-  const float s = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-  dx[i] = s * g[0] * t.scale[k];
+  const float gs = g[0] * t.scale[k];
+  const int64_t base = (int64_t)(blockIdx.x - t.first_block[k]) * kBwdItems + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kBwdItems / kThreads; ++r) {
+    const int64_t i = base + r * kThreads;
+    if (i < t.n[k]) {
+      const float d = t.x[k][i] - t.y[k][i];
+      dx[i] = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * gs;
+    }
+  }
 }
 
 // Builds the launch table; blocks_of(n) = the blocks a term of n elements
@@ -164,7 +175,7 @@ int gsr_l1_sum_bwd(const GsrL1Terms *terms, const float *g_out, void *stream) {
   GSR_REQUIRE(terms != nullptr && g_out != nullptr, "l1_sum_bwd: no term table / gradient");
   L1Table t;
   const int nb = l1_table(
-      *terms, [](int64_t n) -> int64_t { return (n + kThreads - 1) / kThreads; }, t);
+      *terms, [](int64_t n) -> int64_t { return (n + kBwdItems - 1) / kBwdItems; }, t);
   GSR_REQUIRE(nb > 0, "l1_sum_bwd: bad term table (1..%d non-empty terms)", GSR_MAX_L1_TERMS);
   l1_sum_bwd_kernel<<<nb, kThreads, 0, (hipStream_t)stream>>>(t, g_out);
   return gsr_launch_status("l1_sum_bwd");
