@@ -191,7 +191,11 @@ __device__ __forceinline__ void blend_bwd_tile(
     const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
     int ld, BlendLds<C, BD> &sm, float *__restrict__ d_uv_sep = nullptr,
     float *__restrict__ d_op_sep = nullptr) {
+  constexpr int NW = TW * TH / 64;  // waves (strips) per tile
+  constexpr int RW = 64 / TW;       // pixel rows per strip
   static_assert(BD <= TW * TH, "batch depth exceeds the workgroup");
+  static_assert(BD <= 256, "batch slots are stored as bytes");
+  static_assert(64 % TW == 0 && (TW * TH) % 64 == 0 && NW <= kMaxTileWaves, "tile shape");
   constexpr int QP = GradLayout<C>::QP;
   constexpr int Q = GradLayout<C>::Q;
   const int tx = tv.tile % tv.gx, ty = tv.tile / tv.gx;
@@ -201,6 +205,10 @@ __device__ __forceinline__ void blend_bwd_tile(
   const float fpx = (float)px, fpy = (float)py;
   const int64_t pix = tv.pix_base + (int64_t)py * tv.W + px;
   const int lane = lane_id();
+  const int wave = (int)threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  const float bx0 = (float)(tx * TW), bx1 = (float)(min(tx * TW + TW, tv.W) - 1);
+  const int by0 = ty * TH, byl = min(ty * TH + TH, tv.H) - 1;
   const int my_q = bfly_index<QP>(lane);
   const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
   // record column of this lane's quantity: compact rows [da, db, dc,
@@ -246,19 +254,54 @@ __device__ __forceinline__ void blend_bwd_tile(
     const int bstart = max(start, bend - BD);
     __syncthreads();
     const int j = bstart + (int)threadIdx.x;
+    unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < bend) {
       const int g = ids[j];
+      const float u = uv[2 * (int64_t)g], v = uv[2 * (int64_t)g + 1];
+      const float ca = conic[3 * (int64_t)g], cb = conic[3 * (int64_t)g + 1],
+                  cc = conic[3 * (int64_t)g + 2];
+      const float o = opacity[g - tv.op_offset];
       SplatLds sp;
-      sp.xy = make_float2(uv[2 * (int64_t)g], uv[2 * (int64_t)g + 1]);
-      sp.co = make_float4(-0.5f * conic[3 * (int64_t)g], -conic[3 * (int64_t)g + 1],
-                          -0.5f * conic[3 * (int64_t)g + 2], opacity[g - tv.op_offset]);
+      sp.xy = make_float2(u, v);
+      sp.co = make_float4(-0.5f * ca, -cb, -0.5f * cc, o);
       sm.sp[threadIdx.x] = sp;
       sm.id[threadIdx.x] = g;
 #pragma unroll
       for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
+      // per-strip lists as in the forward (a splat no pixel of the strip can
+      // see at alpha >= kAlphaMin changed nothing there)
+      bool reach;
+      const TightSplat ts = tight_setup(u, v, ca, cb, cc, o, reach);
+      if (reach) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const int y0 = by0 + w * RW, y1 = min(y0 + RW - 1, byl);
+          if (y0 <= y1 && tile_reachable(ts, bx0, bx1, (float)y0, (float)y1)) mask |= 1u << w;
+        }
+      }
+    }
+    int pos[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint64_t bal = __ballot((mask >> w) & 1u);
+      pos[w] = (int)__popcll(bal & lt);
+      if (lane == 0) sm.wcnt[w][wave] = (int)__popcll(bal);
     }
     __syncthreads();
-    for (int k = bend - bstart - 1; k >= 0; --k) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if ((mask >> w) & 1u) {
+        int off = pos[w];
+        for (int q = 0; q < wave; ++q) off += sm.wcnt[w][q];
+        sm.list[w][off] = (uint8_t)threadIdx.x;
+      }
+    }
+    int n_mine = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) n_mine += sm.wcnt[wave][q];
+    __syncthreads();
+    for (int i = n_mine - 1; i >= 0; --i) {  // the strip's list, back to front
+      const int k = __builtin_amdgcn_readfirstlane((int)sm.list[wave][i]);
       const int local = bstart + k - start;
       float v[QP];
 #pragma unroll
