@@ -358,8 +358,9 @@ int gsr_uv_grad_scale(const float *d_splat, int ld, int64_t n, float sx, float s
  * scale[N,3] = exp(log_scale), rot[N,4] = quat / max(|quat|, 1e-12),
  * opacity[N] = sigmoid(logit).  Backward: gradients of the three outputs
  * (each may be NULL = zero) -> d_log_scale, d_quat, d_logit (each may be NULL
- * = not wanted), from the forward's inputs and outputs.  quat, rot, g_rot and
- * d_quat are 16-byte aligned. */
+ * = not wanted), from the forward's inputs and outputs; rot is not read (the
+ * unit quaternion is recomputed from quat) and may be NULL.  quat, rot, g_rot
+ * and d_quat are 16-byte aligned. */
 int gsr_activate_fwd(const float *log_scale, const float *quat, const float *logit, int64_t n,
                      float *scale, float *rot, float *opacity, void *stream);
 int gsr_activate_bwd(const float *quat, const float *scale, const float *rot,

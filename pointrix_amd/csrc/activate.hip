@@ -27,8 +27,10 @@ __global__ __launch_bounds__(kThreads) void activate_fwd_kernel(
 
 // d log_scale = g_s * s;  d q = (g_r - r (r . g_r)) / |q|  (g_r / 1e-12 where
 // the norm is clamped);  d logit = g_o * o (1 - o).  A NULL gradient is zero.
+// r is recomputed from q with the forward's operations (bit-identical), which
+// saves reading it back.
 __global__ __launch_bounds__(kThreads) void activate_bwd_kernel(
-    const float *__restrict__ q, const float *__restrict__ s, const float *__restrict__ r,
+    const float *__restrict__ q, const float *__restrict__ s,
     const float *__restrict__ o, const float *__restrict__ gs, const float *__restrict__ gr,
     const float *__restrict__ go, int64_t n, float *__restrict__ dls, float *__restrict__ dq,
     float *__restrict__ dlo) {
@@ -42,8 +44,9 @@ __global__ __launch_bounds__(kThreads) void activate_bwd_kernel(
     float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
     if (gr != nullptr) g = reinterpret_cast<const float4 *>(gr)[i];
     const float4 v = reinterpret_cast<const float4 *>(q)[i];
-    const float4 u = reinterpret_cast<const float4 *>(r)[i];
     const float nq = sqrtf(((v.x * v.x + v.y * v.y) + v.z * v.z) + v.w * v.w);
+    const float dn = fmaxf(nq, 1e-12f);
+    const float4 u = make_float4(v.x / dn, v.y / dn, v.z / dn, v.w / dn);
     float4 d;
     if (nq > 1e-12f) {
       const float dot = ((u.x * g.x + u.y * g.y) + u.z * g.z) + u.w * g.w;
@@ -83,7 +86,7 @@ int gsr_activate_bwd(const float *quat, const float *scale, const float *rot,
               "activate_bwd: quaternion arrays must be 16-byte aligned");
   if (n == 0) return GSR_OK;
   activate_bwd_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, (hipStream_t)stream>>>(
-      quat, scale, rot, opacity, g_scale, g_rot, g_opacity, n, d_log_scale, d_quat, d_logit);
+      quat, scale, opacity, g_scale, g_rot, g_opacity, n, d_log_scale, d_quat, d_logit);
   return gsr_launch_status("activate_bwd");
 }
 

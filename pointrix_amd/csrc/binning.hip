@@ -207,7 +207,21 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   __syncthreads();
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
-  if (base < n) {
+  if (base + kTileItems <= n) {
+    // full block: 16-byte loads, all in flight before the LDS counts
+    // (keys and the block base are 16-byte aligned)
+    uint4 v[kItems / 4];
+#pragma unroll
+    for (int k = 0; k < kItems / 4; ++k)
+      v[k] = reinterpret_cast<const uint4 *>(keys + base)[k * kThreads + threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kItems / 4; ++k) {
+      atomicAdd(&s_h[(v[k].x >> shift) & mask], 1u);
+      atomicAdd(&s_h[(v[k].y >> shift) & mask], 1u);
+      atomicAdd(&s_h[(v[k].z >> shift) & mask], 1u);
+      atomicAdd(&s_h[(v[k].w >> shift) & mask], 1u);
+    }
+  } else if (base < n) {
 #pragma unroll 4
     for (int k = 0; k < kItems; ++k) {
       const uint32_t i = base + k * kThreads + threadIdx.x;

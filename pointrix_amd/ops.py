@@ -783,14 +783,14 @@ class _Activate(torch.autograd.Function):
         s, r, o = torch.empty_like(ls), torch.empty_like(q), torch.empty_like(lo)
         call("gsr_activate_fwd", ptr(ls), ptr(q), ptr(lo), n, ptr(s), ptr(r), ptr(o),
              stream_of(ls))
-        ctx.save_for_backward(q, s, r, o)
+        ctx.save_for_backward(q, s, o)  # the backward recomputes r from q
         ctx.shapes = (log_scale.shape, quat.shape, logit.shape)
         ctx.set_materialize_grads(False)
         return s.view(log_scale.shape), r.view(quat.shape), o.view(logit.shape)
 
     @staticmethod
     def backward(ctx, gs, gr, go):
-        q, s, r, o = ctx.saved_tensors
+        q, s, o = ctx.saved_tensors
         n = q.shape[0]
         need = ctx.needs_input_grad
         d_ls = torch.empty_like(s) if need[0] else None
@@ -799,7 +799,7 @@ class _Activate(torch.autograd.Function):
         gs = None if gs is None else _f32(gs)
         gr = None if gr is None else _aligned_f32(gr)
         go = None if go is None else _f32(go)
-        call("gsr_activate_bwd", ptr(q), ptr(s), ptr(r), ptr(o), ptr(gs), ptr(gr), ptr(go), n,
+        call("gsr_activate_bwd", ptr(q), ptr(s), None, ptr(o), ptr(gs), ptr(gr), ptr(go), n,
              ptr(d_ls), ptr(d_q), ptr(d_lo), stream_of(q))
         shp = ctx.shapes
         return (None if d_ls is None else d_ls.view(shp[0]),
