@@ -282,12 +282,22 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   int32_t val[kItems];
   uint32_t rank[kItems];
   const uint32_t wbase = base + w * (kTileItems / (kThreads / 64));
+  // every load of the block issued before the first ranking step (the
+  // ranking's wave barriers would otherwise expose one load latency per row),
+  // with this thread's digit total and block offset
+  const uint32_t my_total = (int)threadIdx.x < (1 << bits) ? totals[threadIdx.x] : 0u;
+  const uint32_t my_hist = hist_scanned[threadIdx.x * nb + blockIdx.x];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t i = wbase + r * 64 + lane;
     const bool valid = i < n;
     key[r] = valid ? kin[i] : 0u;
     val[r] = valid ? vin[i] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = wbase + r * 64 + lane;
+    const bool valid = i < n;
     const uint32_t d = (key[r] >> shift) & mask;
     uint64_t m = __ballot(valid);
 #pragma unroll
@@ -319,10 +329,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     uint32_t total;
     const uint32_t lstart = block_exclusive_scan(run, total);
     // global start of digit d: all smaller digits, then d in earlier blocks
-    const uint32_t dstart = block_exclusive_scan(d < (1u << bits) ? totals[d] : 0u, total);
+    const uint32_t dstart = block_exclusive_scan(my_total, total);
 #pragma unroll
     for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][d] += lstart;
-    s_delta[d] = (int32_t)(dstart + hist_scanned[d * nb + blockIdx.x] - lstart);
+    s_delta[d] = (int32_t)(dstart + my_hist - lstart);
   }
   __syncthreads();
 #pragma unroll
