@@ -579,59 +579,95 @@ struct StoreCompact {
   }
 };
 
-// Emission, wave-cooperative: lane l of a wave loads depth-sorted splat
-// (wave base + l) -- its tile rectangle as the count kernel packed it, one
-// 8-byte gather per splat; the wave then walks its 64 splats in order and,
-// for each, all 64 lanes write consecutive instances (coalesced stores, no
-// per-splat imbalance).  Row/column of instance k within the w-wide
-// rectangle: q = trunc((k + 0.5) / w) in float is exact while k < 2^22 / w
-// (k is never within 0.5/w of a multiple of w), far beyond the largest tile
-// grid.
+// Emission, wave-cooperative over instances: lane l of a wave loads
+// depth-sorted splat (wave base + l) -- its tile rectangle as the count kernel
+// packed it, one 8-byte gather -- and the wave then writes the instance range
+// of its 64 splats 64 consecutive instances at a time (coalesced stores; a
+// round serves as many small splats as start in it, where a round per splat
+// would leave most lanes idle).  The owner of instance c0 + l is found without
+// a search: the splat holding c0 is the last lane whose offset is <= c0, and
+// each later owner starts at a chunk position that its lane flags in LDS, so
+// the owner is that splat plus the number of flagged positions in [1, l].
+// Row/column of instance k within the w-wide rectangle: q = trunc((k + 0.5)
+// / w) in float is exact while k < 2^22 / w (k is never within 0.5/w of a
+// multiple of w), far beyond the largest tile grid.
 __global__ __launch_bounds__(kThreads) void emit_kernel(
     const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
     const uint2 *__restrict__ rects, const uint32_t *nv_ptr, uint32_t cap, Grid g,
     uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
+  __shared__ uint32_t s_flag[kThreads];
   const uint32_t nv = count_of(nv_ptr, cap);
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
   if (wave_base >= nv) return;  // uniform per wave
   const int lane = lane_id();
+  uint32_t *flag = s_flag + (threadIdx.x & ~63u);
   const uint32_t s = wave_base + lane;
-  int id = 0, x0 = 0, y0 = 0, w = 1, cnt = 0, gxi = 0, toff = 0;
+  const bool mine = s < nv;
+  int id = 0, toff = 0;
+  uint32_t xy = 0, wgx = 1, off = 0xFFFFFFFFu, end = 0;
   float inv_w = 1.0f;
-  uint32_t off = 0;
-  if (s < nv) {
+  if (mine) {
     id = sorted_ids[s];
     const ItemGrid ig = item_grid(g, (uint32_t)id);
-    gxi = ig.gx;
     toff = ig.toff;
     const uint2 r = rects[id];
-    x0 = (int)(r.x & 0xFFFFu);
-    y0 = (int)(r.x >> 16);
-    const int rw = (int)(r.y & 0xFFFFu), rh = (int)(r.y >> 16);
-    w = max(rw, 1);
+    xy = r.x;  // x0 | y0 << 16
+    const uint32_t rw = r.y & 0xFFFFu, rh = r.y >> 16;
+    const uint32_t w = rw > 1u ? rw : 1u;
+    wgx = w | ((uint32_t)ig.gx << 16);  // the tile grid is < 65536 per side
     inv_w = 1.0f / (float)w;
-    cnt = rw * rh;
     off = offsets[s];
+    end = off + rw * rh;
   }
   const int nsplat = min(64, (int)(nv - wave_base));
-  for (int j = 0; j < nsplat; ++j) {
-    const int jid = __builtin_amdgcn_readlane(id, j);
-    const int jx0 = __builtin_amdgcn_readlane(x0, j);
-    const int jy0 = __builtin_amdgcn_readlane(y0, j);
-    const int jw = __builtin_amdgcn_readlane(w, j);
-    const int jcnt = __builtin_amdgcn_readlane(cnt, j);
-    const int jgx = __builtin_amdgcn_readlane(gxi, j);
-    const int jtoff = __builtin_amdgcn_readlane(toff, j);
-    const float jinv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inv_w), j));
-    const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j);
-    // capacity guard: a caller that sized the instance buffers below the
-    // device count (graph replays) gets a truncated, in-bounds emission
-    const int kmax = joff >= cap_inst ? 0 : (int)min((uint32_t)jcnt, cap_inst - joff);
-    for (int k = lane; k < kmax; k += 64) {
-      const int row = (int)(((float)k + 0.5f) * jinv);
-      const int col = k - row * jw;
-      tkeys[joff + k] = (uint32_t)(jtoff + (jy0 + row) * jgx + (jx0 + col));
-      tvals[joff + k] = jid;
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)off, 0);
+  const uint32_t r1 = min((uint32_t)__builtin_amdgcn_readlane((int)end, nsplat - 1), cap_inst);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  // capacity guard: a caller that sized the instance buffers below the device
+  // count (graph replays) gets a truncated, in-bounds emission
+  for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
+    const int j0 = 63 - __clzll(__ballot(off <= c0));  // splat holding instance c0
+    const uint32_t jend = min((uint32_t)__builtin_amdgcn_readlane((int)end, j0), r1);
+    if (jend >= c0 + 64u) {
+      // whole chunks inside splat j0 (uniform): no owner search
+      const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j0);
+      const uint32_t jxy = (uint32_t)__builtin_amdgcn_readlane((int)xy, j0);
+      const uint32_t jwgx = (uint32_t)__builtin_amdgcn_readlane((int)wgx, j0);
+      const int jtoff = __builtin_amdgcn_readlane(toff, j0), jid = __builtin_amdgcn_readlane(id, j0);
+      const float jinv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inv_w), j0));
+      const int w = (int)(jwgx & 0xFFFFu), gx = (int)(jwgx >> 16);
+      const int x0 = (int)(jxy & 0xFFFFu), y0 = (int)(jxy >> 16);
+      for (; c0 + 64u <= jend; c0 += 64) {
+        const uint32_t q = c0 + (uint32_t)lane;
+        const int k = (int)(q - joff);
+        const int row = (int)(((float)k + 0.5f) * jinv);
+        const int col = k - row * w;
+        tkeys[q] = (uint32_t)(jtoff + (y0 + row) * gx + (x0 + col));
+        tvals[q] = jid;
+      }
+      c0 -= 64;  // the loop increment moves to the first chunk not yet written
+      continue;
+    }
+    flag[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t p = off - c0;  // chunk position of this splat's first instance
+    if (p >= 1u && p < 64u) flag[p] = 1u;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t starts = __ballot(flag[lane] != 0u);
+    __builtin_amdgcn_wave_barrier();
+    const int o = j0 + (int)__popcll(starts & upto);
+    const uint32_t ooff = (uint32_t)shfl_i((int)off, o);
+    const uint32_t oxy = (uint32_t)shfl_i((int)xy, o), owgx = (uint32_t)shfl_i((int)wgx, o);
+    const int otoff = shfl_i(toff, o), oid = shfl_i(id, o);
+    const float oinv = shfl_f(inv_w, o);
+    const uint32_t q = c0 + (uint32_t)lane;
+    if (q < r1) {
+      const int k = (int)(q - ooff);
+      const int w = (int)(owgx & 0xFFFFu), gx = (int)(owgx >> 16);
+      const int row = (int)(((float)k + 0.5f) * oinv);
+      const int col = k - row * w;
+      tkeys[q] = (uint32_t)(otoff + ((int)(oxy >> 16) + row) * gx + ((int)(oxy & 0xFFFFu) + col));
+      tvals[q] = oid;
     }
   }
 }
