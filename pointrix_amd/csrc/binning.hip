@@ -505,11 +505,15 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
   uint32_t c = 0;
   if (i < n) {
     const int r = radius[i];
+    // the position is requested with the radius (one memory round trip); the
+    // empty asm consumes it here so the compiler cannot sink the load into
+    // the branch
+    const float pu = uv[2 * (int64_t)i], pv = uv[2 * (int64_t)i + 1];
+    asm volatile("" : : "v"(pu), "v"(pv));
     uint2 packed = make_uint2(0u, 0u);
     if (r > 0) {
       const ItemGrid ig = item_grid(g, i);
-      const Rect q =
-          tile_rect(uv[2 * (int64_t)i], uv[2 * (int64_t)i + 1], r, ig.gx, ig.gy, g.tw, g.th);
+      const Rect q = tile_rect(pu, pv, r, ig.gx, ig.gy, g.tw, g.th);
       const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
       c = w * h;
       packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));

@@ -92,14 +92,16 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
     float *__restrict__ opac_act) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (opac_act != nullptr) opac_act[i] = act_opacity(opac_raw[i]);
   const CamV cam = cam_load(intr, extr);
   const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  // requested with the position; stored once both have arrived
+  const float o_raw = opac_act != nullptr ? opac_raw[i] : 0.0f;
   float u, v, d;
   project(cam, x, nearest, extent, W, H, u, v, d);
   uv[2 * i] = u;
   uv[2 * i + 1] = v;
   depth[i] = d;
+  if (opac_act != nullptr) opac_act[i] = act_opacity(o_raw);
   float con[3] = {0.f, 0.f, 0.f};
   float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -199,6 +201,10 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
 #pragma unroll
     for (int j = 0; j < KE * 3; ++j) dsh[j] = 0.0f;
     const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    // inputs that do not depend on visibility are requested with the position
+    const float g_depth_in = d_depth != nullptr ? d_depth[i] : 0.0f;
+    const float o_raw = d_opac_raw != nullptr ? opac_raw[i] : 0.0f;
+    const float g_oact = d_opac_raw != nullptr ? d_opac_act[i * ld_oact] : 0.0f;
     float3 s_act = make_float3(1.f, 1.f, 1.f);
     float4 q_act = make_float4(1.f, 0.f, 0.f, 0.f);
     float q_norm = 1.0f;
@@ -257,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       }
       // channel layout rgb | depth (flag 1) | normal (flag 2)
       const bool has_d = (flags & kFlagDepth) != 0;
-      float gdepth = d_depth ? d_depth[i] : 0.0f;
+      float gdepth = g_depth_in;
       if (has_d) gdepth += gf[3];
       if (flags & kFlagNormal) {
         NormalFwd nf;
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       // projected but not rasterised: only a direct depth gradient applies
       float u, v, dd;
       project(cam, x, nearest, extent, W, H, u, v, dd);
-      if (dd != 0.0f) project_vjp(cam, x, 0.0f, 0.0f, d_depth[i], gx, gcam);
+      if (dd != 0.0f) project_vjp(cam, x, 0.0f, 0.0f, g_depth_in, gx, gcam);
     }
     d_xyz[3 * i] = gx.x;
     d_xyz[3 * i + 1] = gx.y;
@@ -303,8 +309,8 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         for (int k = 0; k < 4; ++k) gq[k] = gq[k] / 1e-12f;
       }
       if (d_opac_raw != nullptr) {
-        const float a = act_opacity(opac_raw[i]);
-        d_opac_raw[i] = d_opac_act[i * ld_oact] * (a * (1.0f - a));
+        const float a = act_opacity(o_raw);
+        d_opac_raw[i] = g_oact * (a * (1.0f - a));
       }
     }
 #pragma unroll
