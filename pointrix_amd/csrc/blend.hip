@@ -33,21 +33,21 @@ __global__ __launch_bounds__(TW *TH) void blend_fwd_kernel(
                             feat, ids, bg, out, Tfin, ncon, sm);
 }
 
-// Two pixels per lane: TW*TH/2 threads, batches of TW*TH/2 splats.
-template <int C, int TW, int TH>
-__global__ __launch_bounds__(TW *TH / 2) void blend_bwd2_kernel(
+// PP pixels per lane: TW*TH/PP threads, batches of TW*TH/PP splats.
+template <int C, int TW, int TH, int PP>
+__global__ __launch_bounds__(TW *TH / PP) void blend_bwd2_kernel(
     const float *__restrict__ uv, const float *__restrict__ conic,
     const float *__restrict__ opacity, const float *__restrict__ feat,
     const int32_t *__restrict__ ids, const int32_t *__restrict__ ranges, float bg, int W, int H,
     int gx, const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
     const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
     int ld, float *__restrict__ d_uv, float *__restrict__ d_op) {
-  __shared__ BlendLds<C, TW * TH / 2> sm;
+  __shared__ BlendLds<C, TW * TH / PP> sm;
   const int tile = blockIdx.x;
   const TileView tv{W, H, gx, tile, 0, (int64_t)H * W, 0};
-  blend_bwd_tile2<C, TW, TH, TW * TH / 2>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic,
-                                          opacity, feat, ids, bg, Tfin, ncon, dout, dTfin,
-                                          d_splat, ld, sm, d_uv, d_op);
+  blend_bwd_tile2<C, TW, TH, TW * TH / PP, PP>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv,
+                                               conic, opacity, feat, ids, bg, Tfin, ncon, dout,
+                                               dTfin, d_splat, ld, sm, d_uv, d_op);
 }
 
 template <int C, int TW, int TH>
@@ -97,7 +97,7 @@ int launch_bwd(const float *uv, const float *conic, const float *opacity, const 
   uv, conic, opacity, feature, ids, tile_range, bg, W, H, gx, T_final, n_contrib, d_out, \
       d_T_final, d_splat, ld, d_uv, d_op
   if (tw == 16 && th == 16)
-    blend_bwd2_kernel<C, 16, 16><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
+    blend_bwd2_kernel<C, 16, 16, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 8 && th == 32)
     blend_bwd_kernel<C, 8, 32><<<nt, 256, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 32 && th == 8)
