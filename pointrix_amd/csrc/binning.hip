@@ -241,14 +241,26 @@ __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restri
   uint32_t *row = hist + (size_t)blockIdx.x * nb;
   const uint32_t per = (nb + kThreads - 1) / kThreads;
   const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  // the run is read in groups of 8 independent loads (one latency per group)
   uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; ++b) sum += row[b];
+  for (uint32_t b = b0; b < b1; b += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? row[b + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += v[k];
+  }
   uint32_t total;
   uint32_t run = block_exclusive_scan(sum, total);
-  for (uint32_t b = b0; b < b1; ++b) {
-    const uint32_t v = row[b];
-    row[b] = run;
-    run += v;
+  for (uint32_t b = b0; b < b1; b += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = b + k < b1 ? row[b + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (b + k < b1) row[b + k] = run;
+      run += v[k];
+    }
   }
   if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
