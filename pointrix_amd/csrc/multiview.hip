@@ -375,8 +375,10 @@ __global__ __launch_bounds__(TW *TH) void mv_blend_fwd_kernel(
   }
 }
 
+// Two pixels per lane (TW*TH/2 threads), as the one-view 16x16 backward; the
+// batch depth is capped at the thread count.
 template <int C, int TW, int TH, int BD>
-__global__ __launch_bounds__(TW *TH) void mv_blend_bwd_kernel(
+__global__ __launch_bounds__(TW *TH / 2) void mv_blend_bwd_kernel(
     const float *__restrict__ uv, const float *__restrict__ conic,
     const float *__restrict__ opacity, const float *__restrict__ feat,
     const int32_t *__restrict__ ids, const int32_t *__restrict__ ranges, float bg, int V,
@@ -384,7 +386,8 @@ __global__ __launch_bounds__(TW *TH) void mv_blend_bwd_kernel(
     int *__restrict__ queue, const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
     const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
     int ld) {
-  __shared__ BlendLds<C, BD> sm;
+  constexpr int BDX = BD < TW * TH / 2 ? BD : TW * TH / 2;
+  __shared__ BlendLds<C, BDX> sm;
   __shared__ int s_vi[kMaxViews * kVI];
   __shared__ int s_tile;
   for (int k = threadIdx.x; k < V * kVI; k += blockDim.x) s_vi[k] = vi[k];
@@ -395,9 +398,9 @@ __global__ __launch_bounds__(TW *TH) void mv_blend_bwd_kernel(
     const int tile = s_tile;
     if (tile >= T_total) break;  // uniform
     const TileView tv = mv_tile_view(s_vi, V, tile, npv, P_total);
-    blend_bwd_tile<C, TW, TH, BD>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic,
-                                  opacity, feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat, ld,
-                                  sm);
+    blend_bwd_tile2<C, TW, TH, BDX, 2>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic,
+                                       opacity, feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat,
+                                       ld, sm, nullptr, nullptr);
   }
 }
 
@@ -459,7 +462,7 @@ int dispatch_blend(int C, int tw, int th, int bd, unsigned nb, hipStream_t st, A
 #define GSR_K(C_, TW_, TH_, BD_)                                                        \
   if (C == C_ && tw == TW_ && th == TH_ && bd == BD_) {                                 \
     if constexpr (FWD) mv_blend_fwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_, 0, st>>>(a...); \
-    else mv_blend_bwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_, 0, st>>>(a...);         \
+    else mv_blend_bwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_ / 2, 0, st>>>(a...);    \
     return GSR_OK;                                                                      \
   }
 #define GSR_SHAPES(C_)                                                                  \
@@ -546,7 +549,8 @@ int gsr_mv_alpha_blending_bwd(const float *uv, const float *conic, const float *
   if (total_tiles == 0) return GSR_OK;
   hipStream_t st = (hipStream_t)stream;
   zero_int_kernel<<<1, 1, 0, st>>>(queue);
-  const unsigned nb = (unsigned)min(total_tiles, persistent_blocks());
+  // half-size workgroups: twice as many keep the same waves per CU
+  const unsigned nb = (unsigned)min(total_tiles, 2 * persistent_blocks());
   const int rc = dispatch_blend<false>(C, tile_w, tile_h, batch_depth, nb, st, uv, conic,
                                        opacity, feature, ids, tile_range, bg, V, view_i,
                                        n_per_view, total_tiles, total_pixels, queue, T_final,
