@@ -50,21 +50,6 @@ __global__ __launch_bounds__(TW *TH / PP) void blend_bwd2_kernel(
                                                dTfin, d_splat, ld, sm, d_uv, d_op);
 }
 
-template <int C, int TW, int TH>
-__global__ __launch_bounds__(TW *TH) void blend_bwd_kernel(
-    const float *__restrict__ uv, const float *__restrict__ conic,
-    const float *__restrict__ opacity, const float *__restrict__ feat,
-    const int32_t *__restrict__ ids, const int32_t *__restrict__ ranges, float bg, int W, int H,
-    int gx, const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
-    const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
-    int ld, float *__restrict__ d_uv, float *__restrict__ d_op) {
-  __shared__ BlendLds<C, TW * TH> sm;
-  const int tile = blockIdx.x;
-  const TileView tv{W, H, gx, tile, 0, (int64_t)H * W, 0};
-  blend_bwd_tile<C, TW, TH, TW * TH>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic, opacity,
-                            feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat, ld, sm, d_uv, d_op);
-}
-
 template <int C>
 int launch_fwd(const float *uv, const float *conic, const float *opacity, const float *feature,
                const int32_t *ids, const int32_t *tile_range, float bg, int W, int H, int tw,
@@ -99,9 +84,9 @@ int launch_bwd(const float *uv, const float *conic, const float *opacity, const 
   if (tw == 16 && th == 16)
     blend_bwd2_kernel<C, 16, 16, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 8 && th == 32)
-    blend_bwd_kernel<C, 8, 32><<<nt, 256, 0, s>>>(GSR_BWD_ARGS);
+    blend_bwd2_kernel<C, 8, 32, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 32 && th == 8)
-    blend_bwd_kernel<C, 32, 8><<<nt, 256, 0, s>>>(GSR_BWD_ARGS);
+    blend_bwd2_kernel<C, 32, 8, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else
     return GSR_ERR_ARG;
 #undef GSR_BWD_ARGS

@@ -182,190 +182,17 @@ struct GradLayout {
   static constexpr int QP = Q <= 8 ? 8 : (Q <= 16 ? 16 : 32);  // butterfly width
 };
 
-template <int C, int TW, int TH, int BD>
-__device__ __forceinline__ void blend_bwd_tile(
-    const TileView &tv, int start, int end, const float *__restrict__ uv,
-    const float *__restrict__ conic, const float *__restrict__ opacity,
-    const float *__restrict__ feat, const int32_t *__restrict__ ids, float bg,
-    const float *__restrict__ Tfin, const int32_t *__restrict__ ncon,
-    const float *__restrict__ dout, const float *__restrict__ dTfin, float *__restrict__ d_splat,
-    int ld, BlendLds<C, BD> &sm, float *__restrict__ d_uv_sep = nullptr,
-    float *__restrict__ d_op_sep = nullptr) {
-  constexpr int NW = TW * TH / 64;  // waves (strips) per tile
-  constexpr int RW = 64 / TW;       // pixel rows per strip
-  static_assert(BD <= TW * TH, "batch depth exceeds the workgroup");
-  static_assert(BD <= 256, "batch slots are stored as bytes");
-  static_assert(64 % TW == 0 && (TW * TH) % 64 == 0 && NW <= kMaxTileWaves, "tile shape");
-  constexpr int QP = GradLayout<C>::QP;
-  constexpr int Q = GradLayout<C>::Q;
-  const int tx = tv.tile % tv.gx, ty = tv.tile / tv.gx;
-  const int px = tx * TW + (int)threadIdx.x % TW;
-  const int py = ty * TH + (int)threadIdx.x / TW;
-  const bool inside = px < tv.W && py < tv.H;
-  const float fpx = (float)px, fpy = (float)py;
-  const int64_t pix = tv.pix_base + (int64_t)py * tv.W + px;
-  const int lane = lane_id();
-  const int wave = (int)threadIdx.x >> 6;
-  const uint64_t lt = lanemask_lt();
-  const float bx0 = (float)(tx * TW), bx1 = (float)(min(tx * TW + TW, tv.W) - 1);
-  const int by0 = ty * TH, byl = min(ty * TH + TH, tv.H) - 1;
-  const int my_q = bfly_index<QP>(lane);
-  const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
-  // record column of this lane's quantity: compact rows [da, db, dc,
-  // dfeature...] when both dense arrays are given, else the full row
-  const int rec_col = (d_uv_sep != nullptr && d_op_sep != nullptr)
-                          ? my_q - (my_q < 5 ? 2 : 3)
-                          : my_q;
-
-  float T = 1.0f, gpix[C];
-  int last = 0;
-  float gT = 0.0f;
-  if (inside) {
-    T = Tfin[pix];
-    last = ncon[pix];
-#pragma unroll
-    for (int c = 0; c < C; ++c) gpix[c] = dout[c * tv.chan_stride + pix];
-    if (dTfin != nullptr) gT = dTfin[pix];
-  } else {
-#pragma unroll
-    for (int c = 0; c < C; ++c) gpix[c] = 0.0f;
-  }
-  __syncthreads();  // sm.misc may still be read by the previous tile
-  if (threadIdx.x == 0) sm.misc = 0;
-  __syncthreads();
-  if (last > 0) atomicMax(&sm.misc, last);
-  __syncthreads();
-  const int stop = start + sm.misc;
-  if (stop <= start || end <= start) return;  // uniform
-  const float T_final = T;
-  float gsum = 0.0f;
-#pragma unroll
-  for (int c = 0; c < C; ++c) gsum += gpix[c];
-  const float bg_term = bg * gsum + gT;  // d out / d T_final summed with d T_final
-  // dL/dalpha_i = T_i sum_c (f_ic - A_ic) g_c - ..., where A_c (the colour
-  // blended behind splat i) obeys A <- a' f' + (1 - a') A over the splats
-  // behind it (a', f' the next splat back).  The recurrence is linear, so
-  // the pixel keeps only the scalars S = sum_c A_c g_c and F = sum_c f'_c g_c
-  // instead of 2C per-channel values.
-  float S = 0.0f, F = 0.0f;
-  float last_alpha = 0.0f;
-
-  for (int bend = stop; bend > start; bend -= BD) {
-    const int bstart = max(start, bend - BD);
-    __syncthreads();
-    const int j = bstart + (int)threadIdx.x;
-    unsigned mask = 0u;
-    if ((int)threadIdx.x < BD && j < bend) {
-      const int g = ids[j];
-      const float u = uv[2 * (int64_t)g], v = uv[2 * (int64_t)g + 1];
-      const float ca = conic[3 * (int64_t)g], cb = conic[3 * (int64_t)g + 1],
-                  cc = conic[3 * (int64_t)g + 2];
-      const float o = opacity[g - tv.op_offset];
-      SplatLds sp;
-      sp.xy = make_float2(u, v);
-      sp.co = make_float4(-0.5f * ca, -cb, -0.5f * cc, o);
-      sm.sp[threadIdx.x] = sp;
-      sm.id[threadIdx.x] = g;
-#pragma unroll
-      for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
-      // per-strip lists as in the forward (a splat no pixel of the strip can
-      // see at alpha >= kAlphaMin changed nothing there)
-      bool reach;
-      const TightSplat ts = tight_setup(u, v, ca, cb, cc, o, reach);
-      if (reach) {
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-          const int y0 = by0 + w * RW, y1 = min(y0 + RW - 1, byl);
-          if (y0 <= y1 && tile_reachable(ts, bx0, bx1, (float)y0, (float)y1)) mask |= 1u << w;
-        }
-      }
-    }
-    int pos[NW];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const uint64_t bal = __ballot((mask >> w) & 1u);
-      pos[w] = (int)__popcll(bal & lt);
-      if (lane == 0) sm.wcnt[w][wave] = (int)__popcll(bal);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      if ((mask >> w) & 1u) {
-        int off = pos[w];
-        for (int q = 0; q < wave; ++q) off += sm.wcnt[w][q];
-        sm.list[w][off] = (uint8_t)threadIdx.x;
-      }
-    }
-    int n_mine = 0;
-#pragma unroll
-    for (int q = 0; q < NW; ++q) n_mine += sm.wcnt[wave][q];
-    __syncthreads();
-    for (int i = n_mine - 1; i >= 0; --i) {  // the strip's list, back to front
-      const int k = __builtin_amdgcn_readfirstlane((int)sm.list[wave][i]);
-      const int local = bstart + k - start;
-      float v[QP];
-#pragma unroll
-      for (int q = 0; q < QP; ++q) v[q] = 0.0f;
-      // same decisions as the forward, evaluated branch-light
-      const SplatLds sp = sm.sp[k];
-      const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
-      const float power = splat_power(sp.co, dx, dy);
-      const bool near = local < last && !(power > 0.0f) &&
-                        !(power < kPowerCut && sp.co.w <= 1.0f);
-      if (__ballot(near) == 0ull) continue;  // wave-uniform
-      // lanes that do not use this splat evaluate exp at 0 (power may be
-      // large and positive there, and G feeds a product that must stay finite)
-      const float G = exp_neg(near ? power : 0.0f);
-      const float alpha = fminf(kAlphaMax, sp.co.w * G);
-      const bool contrib = near && !(alpha < kAlphaMin);
-      if (__ballot(contrib) == 0ull) continue;  // wave-uniform
-      {
-        const float inv_om = __builtin_amdgcn_rcpf(1.0f - alpha);
-        const float Tn = T * inv_om;
-        T = contrib ? Tn : T;
-        const float w = contrib ? alpha * Tn : 0.0f;
-        float fg = sm.f[k] * gpix[0];
-#pragma unroll
-        for (int c = 1; c < C; ++c) fg = fmaf(sm.f[c * BD + k], gpix[c], fg);
-#pragma unroll
-        for (int c = 0; c < C; ++c) v[6 + c] = w * gpix[c];
-        const float S_n = fmaf(last_alpha, F, (1.0f - last_alpha) * S);
-        S = contrib ? S_n : S;
-        F = contrib ? fg : F;
-        float dL_dalpha = (fg - S_n) * Tn;
-        dL_dalpha -= T_final * inv_om * bg_term;
-        last_alpha = contrib ? alpha : last_alpha;
-        // alpha = min(0.99, o G): no gradient where clamped
-        const float gsel = (contrib && sp.co.w * G < kAlphaMax) ? dL_dalpha : 0.0f;
-        v[5] = G * gsel;
-        const float dpow = alpha * gsel;
-        v[0] = dpow * (2.0f * sp.co.x * dx + sp.co.y * dy);
-        v[1] = dpow * (sp.co.y * dx + 2.0f * sp.co.z * dy);
-        v[2] = dpow * (-0.5f * dx * dx);
-        v[3] = dpow * (-dx * dy);
-        v[4] = dpow * (-0.5f * dy * dy);
-      }
-      Bfly<QP, 32>::run(v, lane);
-      if (writer) {
-        // (du, dv) and dopacity go to their own dense arrays when given
-        const int64_t id = sm.id[k];
-        float *dst = d_splat + id * ld + rec_col;
-        if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
-        if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
-        atomicAdd(dst, v[0]);
-      }
-    }
-  }
-}
-
-// Backward with PP vertically adjacent pixels per lane (TW*TH/PP threads per
-// tile; a wave covers a TW x 64 PP/TW block).  Each lane runs its pixels'
-// reverse recurrences for every splat and adds their contributions before
-// the butterfly, so the cross-lane reduction, the atomic and the LDS reads
-// of the splat are paid once per PP pixels.  PP = 2 is used: at full3m four
-// pixels per lane (one wave per 16x16 tile) measured 358 against 333 us.  Same per-pixel decisions and
-// arithmetic as blend_bwd_tile; only the order in which the per-pixel terms
-// are summed differs.
+// Backward of one tile with PP vertically adjacent pixels per lane
+// (TW*TH/PP threads; a wave covers a TW x 64 PP/TW strip).  Each pixel replays
+// the forward's decisions in reverse from its n_contrib, recovering T_i from
+// T_final.  dL/dalpha_i = T_i sum_c (f_ic - A_ic) g_c - ..., where A_c (the
+// colour blended behind splat i) obeys A <- a' f' + (1 - a') A over the
+// splats behind it (a', f' the next splat back); the recurrence is linear, so
+// a pixel keeps only the scalars S = sum_c A_c g_c and F = sum_c f'_c g_c.
+// Each lane adds its pixels' contributions before the butterfly, so the
+// cross-lane reduction, the atomic and the LDS reads of the splat are paid
+// once per PP pixels.  PP = 2 is used: at full3m four pixels per lane (one
+// wave per 16x16 tile) measured 358 against 333 us.
 template <int C>
 struct PixBwd {
   float T, S, F, last_alpha, T_final, bg_term;
@@ -380,6 +207,8 @@ __device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, i
   const float dx = sp.xy.x - p.fx, dy = sp.xy.y - p.fy;
   const float power = splat_power(sp.co, dx, dy);
   const bool near = local < p.last && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
+  // pixels that do not use this splat evaluate exp at 0 (power may be large
+  // and positive there, and G feeds a product that must stay finite)
   const float G = exp_neg(near ? power : 0.0f);
   const float alpha = fminf(kAlphaMax, sp.co.w * G);
   const bool contrib = near && !(alpha < kAlphaMin);

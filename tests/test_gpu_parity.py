@@ -213,6 +213,43 @@ def test_blend_backward(device, small, pre32, C, bg, with_T):
     assert rel_err(ndc.grad, hl[0].grad * scale) < GRAD_TOL
 
 
+@pytest.mark.parametrize("tile", [(8, 32), (32, 8)])
+def test_blend_tile_shapes(device, small, pre32, tile):
+    """Non-square tiles: the binning and forward bit-exact against the FP32
+    restatement with the same tile shape, the backward against FP64 autograd
+    of the same tiling (within the gradient tolerance)."""
+    inp, cam = small
+    W, H = cam.width, cam.height
+    tw, th = tile
+    ids_r, rng_r = r32.sort_gaussian(pre32["uv"], pre32["depth"], W, H, pre32["radius"], tw, th)
+    t = lambda a: torch.from_numpy(np.asarray(a)).to(device)
+    ids, rng = ops.sort_gaussian(t(pre32["uv"]), t(pre32["depth"]), W, H, t(pre32["radius"]),
+                                 None, tile_w=tw, tile_h=th)
+    assert np.array_equal(ids.cpu().numpy(), ids_r)
+    assert np.array_equal(rng.cpu().numpy(), rng_r)
+    g = torch.Generator().manual_seed(tw)
+    n, C = len(pre32["uv"]), 4
+    feat = torch.rand(n, C, generator=g)
+    gout = torch.randn(C, H, W, generator=g)
+    leaves = [torch.from_numpy(pre32["uv"]), torch.from_numpy(pre32["conic"]),
+              inp["opacity"].clone(), feat]
+    dl = [x.to(device).requires_grad_(True) for x in leaves]
+    out, T, nc = ops.alpha_blending_ex(*dl, ids, rng, 0.0, W, H, None, tw, th)
+    out_r, T_r, nc_r = r32.alpha_blending(pre32["uv"], pre32["conic"], inp["opacity"].numpy(),
+                                          feat.numpy(), ids_r, rng_r, 0.0, W, H, tw, th)
+    assert np.array_equal(out.detach().cpu().numpy(), out_r)
+    assert np.array_equal(nc.cpu().numpy(), nc_r)
+    (out * gout.to(device)).sum().backward()
+    hl = [x.double().requires_grad_(True) for x in leaves]
+    out64 = o64.alpha_blending(*hl, torch.from_numpy(ids_r), torch.from_numpy(rng_r), 0.0, W, H,
+                               tw=tw, th=th)
+    (out64 * gout.double()).sum().backward()
+    for name, a, b in zip(["uv", "conic", "opacity", "feature"], dl, hl):
+        e = rel_err(a.grad, b.grad)
+        print(f"tile {tw}x{th} blend bwd d{name}: rel err {e:.3g}")
+        assert e < GRAD_TOL, name
+
+
 def test_preprocess_backward(device, small, pre32):
     inp, cam = small
     W, H = cam.width, cam.height
