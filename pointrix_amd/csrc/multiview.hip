@@ -203,6 +203,18 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
   float gS[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float gq[4] = {0.f, 0.f, 0.f, 0.f};
   float gop = 0.0f;
+  // the splat's radius in every view, requested together (one memory round
+  // trip instead of one per view), as a bit mask of the views that raster it
+  uint64_t vis = 0;
+  if (active) {
+    for (int v0 = 0; v0 < V; v0 += 16) {
+      int rv[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) rv[k] = v0 + k < V ? radius[(int64_t)(v0 + k) * n + i] : 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) vis |= (uint64_t)(rv[k] > 0) << (v0 + k);
+    }
+  }
   for (int v = 0; v < V; ++v) {
     const float *pv = s_vf + v * kVF;
     const int *iv = s_vi + v * kVI;
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
     for (int k = 0; k < 16; ++k) gcam[k] = 0.0f;
     float gcen[3] = {0.f, 0.f, 0.f};
     const int64_t j = (int64_t)v * n + i;
-    if (active && radius[j] > 0) {
+    if ((vis >> v) & 1u) {
       const CamV cam = cam_from(pv);
       const int W = iv[0], H = iv[1];
       const float *g = rec + j * ld;
