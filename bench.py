@@ -270,7 +270,7 @@ def time_raw(renderer, scene, cam, targets, steps):
 
 
 def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
-    """1M splats (SH 3), 16 cameras of uneven resolution,
+    """Batched workload: 1M splats (SH 3), 16 cameras of uneven resolution,
     summed L1 over views, fwd+bwd.  Plain = one view per launch
     (MsplatRender.render_iter per view, one backward); batched = one launch per
     stage (autotuned tile shape / batch depth); graph = the batched step
@@ -391,7 +391,7 @@ def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
 
 
 def bench_train500k(steps, warmup, log):
-    """500k splats (SH 3), one 979x546 view, loss
+    """Full training step: 500k splats (SH 3), one 979x546 view, loss
     0.8 L1 + 0.2 (1 - SSIM), densification statistics, Adam (pointrix's
     per-attribute learning rates, eps 1e-15): the full training step.
     fused = ops.l1_ssim_loss + DensifyStats + FusedAdam; fused_raw = the same
@@ -548,8 +548,8 @@ def main():
                          "calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the "
+              f"renderer runs on a single GPU (one process, one device)", file=sys.stderr)
         sys.exit(2)
 
     from pointrix_amd import ops

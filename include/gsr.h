@@ -11,7 +11,7 @@
  * hipError_t-derived code on a launch failure, GSR_ERR_ARG (-1000) on a bad
  * argument; gsr_last_error() describes the last failure of the calling thread.
  *
- * Conventions (fixed by this library, documented in DESIGN.md): intrinsics
+ * Conventions (fixed by this library, following 3DGS practice): intrinsics
  * intr = (fx, fy, cx, cy); extrinsics extr = 3x4 row-major world-to-camera
  * [R|t]; pixel centres at integer coordinates; 16x16 tiles unless a tile shape
  * argument says otherwise; cov3d packed (xx, xy, xz, yy, yz, zz); quaternions

@@ -7,8 +7,8 @@ path and, being differentiable through autograd, the gradient target.
 Each function mirrors one msplat call site of the reference renderer
 (pointrix/model/renderer/msplat.py) with the same positional arguments.  msplat
 itself is an un-vendored submodule (.gitmodules:1-3), so its numeric
-conventions are not in the reference tree; the ones fixed here are the
-3DGS-compatible choices documented in DESIGN.md ("Conventions"):
+conventions are not in the reference tree; the ones fixed here are these
+3DGS-compatible choices:
 
 * pinhole uv = (fx*x/z + cx, fy*y/z + cy), pixel centres at integer coords;
 * cull: z <= nearest (0.2 at msplat.py:113) or |2u/W-1| > extent (1.3);
