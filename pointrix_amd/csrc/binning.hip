@@ -275,7 +275,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
-    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ kgather) {
+    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ kgather,
+    uint32_t *__restrict__ rank_out) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
@@ -369,6 +370,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       // carried into the sorted order) instead of the key
       if (kout != nullptr) kout[pos] = kgather != nullptr ? kgather[vv] : kk;
       vout[pos] = vv;
+      // rank_out: each value's sorted position (the inverse permutation)
+      if (rank_out != nullptr) rank_out[vv] = pos;
     }
   }
 }
@@ -688,38 +691,41 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
   }
 }
 
-// Tight emission: the wave's 64 depth-sorted splats own the contiguous
-// output range starting at the first one's offset; their accepted
-// candidates, in order, are compacted into it with a ballot (count_tight
-// accepted exactly the same candidates, so every splat fills
-// [offset, offset + count)).
+// Tight emission in splat index order, as the tight count walked them: the
+// depth order would put the large near splats, with a reachability test per
+// candidate tile, into the same few waves.  A wave's 64 splats' candidates
+// are flattened 64 at a time exactly as in count_tight_kernel; a candidate
+// that passes goes to its splat's depth-order offset (offsets[rank[i]]) plus
+// the passes of that splat before it (earlier rounds: the splat lane's running
+// count; this round: the ballot from the splat's first lane in the round).
+// count_tight accepted exactly the same candidates, so every splat fills
+// [offset, offset + count).
 __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
     const float *__restrict__ uv, const int32_t *__restrict__ radius,
-    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
-    const uint64_t *__restrict__ masks, const uint32_t *nv_ptr, uint32_t cap, Grid g,
+    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ rank,
+    const uint32_t *__restrict__ offsets, const uint64_t *__restrict__ masks, uint32_t n, Grid g,
     uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
-  const uint32_t nv = count_of(nv_ptr, cap);
   const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
-  if (wave_base >= nv) return;  // uniform per wave
+  if (wave_base >= n) return;  // uniform per wave
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
-  const uint32_t s = wave_base + lane;
-  const bool valid = s < nv;
-  const int id = valid ? sorted_ids[s] : 0;
-  const TLane L = tight_lane(g, uv, radius, (uint32_t)id, valid);
+  const uint32_t i = wave_base + lane;
+  const bool valid = i < n;
+  const TLane L = tight_lane(g, uv, radius, i, valid);
   int gxi = 0, toff = 0;
   uint64_t mine = 0;
-  if (valid) {
-    const ItemGrid ig = item_grid(g, (uint32_t)id);
+  uint32_t base = 0;
+  if (valid && L.area > 0) {
+    const ItemGrid ig = item_grid(g, i);
     gxi = ig.gx;
     toff = ig.toff;
-    if (L.area > 0 && L.area <= 64) mine = masks[id];
+    if (L.area <= 64) mine = masks[i];
+    if (cnt[i] > 0) base = offsets[rank[i]];  // splats with no accepted tile are unlisted
   }
   const int mlo = (int)(uint32_t)mine, mhi = (int)(uint32_t)(mine >> 32);
-  const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)offsets[wave_base]);
   int total;
   const int P = wave_excl_scan(L.area, total);
-  uint32_t run = 0;
+  uint32_t done = 0;  // this lane's splat: accepted candidates so far
   for (int c0 = 0; c0 < total; c0 += 64) {
     const uint32_t c = (uint32_t)(c0 + lane);
     const bool in = c < (uint32_t)total;
@@ -738,17 +744,22 @@ __global__ __launch_bounds__(kThreads) void emit_tight_kernel(
       const bool test = owner_test(L, o, tx, ty, g.tw, g.th);
       pass = pass || (in && area_o > 64 && test);
     }
-    const int oid = shfl_i(id, o), ogx = shfl_i(gxi, o), otoff = shfl_i(toff, o);
+    const int ogx = shfl_i(gxi, o), otoff = shfl_i(toff, o);
+    const uint32_t obase = (uint32_t)shfl_i((int)(base + done), o);
     const uint64_t bal = __ballot(pass);
     if (pass) {
-      // capacity guard as in emit_kernel
-      const uint32_t pos = base + run + (uint32_t)__popcll(bal & lt);
-      if (pos < cap_inst) {
+      const int first = max(Po - c0, 0);  // the owner's first lane in this round
+      const uint32_t pos = obase + (uint32_t)__popcll(bal & lt & (~0ull << first));
+      if (pos < cap_inst) {  // capacity guard as in emit_kernel
         tkeys[pos] = (uint32_t)(otoff + ty * ogx + tx);
-        tvals[pos] = oid;
+        tvals[pos] = (int32_t)(wave_base + (uint32_t)o);
       }
     }
-    run += (uint32_t)__popcll(bal);
+    const int lo = max(P, c0) - c0, hi = min(P + L.area, c0 + 64) - c0;
+    if (hi > lo) {
+      const uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+      done += (uint32_t)__popcll((bal & up) >> lo);
+    }
   }
 }
 
@@ -851,7 +862,8 @@ inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBit
 // is returned through the pointers.
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
-               bool final_keys = true, const uint32_t *final_gather = nullptr) {
+               bool final_keys = true, const uint32_t *final_gather = nullptr,
+               uint32_t *final_rank = nullptr) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
@@ -864,7 +876,8 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     const bool last = p == passes - 1;
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         k_cur, v_cur, (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt,
-        n_ptr, cap, shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr);
+        n_ptr, cap, shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr,
+        last ? final_rank : nullptr);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
@@ -962,7 +975,11 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // writes each splat's instance count, in depth order, where the keys would go
   uint32_t *kc = g.sb.keyA;
   int32_t *vc = g.sb.valA;
-  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt);
+  // tight binning also needs each splat's depth rank (its emission walks splats
+  // in index order); the exact count kernel's packed rectangles are not used
+  // there, so their array holds it
+  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt,
+            gr.conic != nullptr ? reinterpret_cast<uint32_t *>(g.rects) : nullptr);
   // offsets of each splat's instances, in depth order -> M in scalars[1]
   run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
            s);
@@ -990,8 +1007,8 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
                                                              (uint32_t)m, ib.keyA, v_first);
   else
     emit_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(
-        uv, radius, g.sb.valA, g.offs, g.masks, g.scalars, (uint32_t)n, gr, (uint32_t)m, ib.keyA,
-        v_first);
+        uv, radius, g.cnt, reinterpret_cast<const uint32_t *>(g.rects), g.offs, g.masks,
+        (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
   run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
