@@ -17,8 +17,8 @@ __global__ __launch_bounds__(kThreads) void activate_fwd_kernel(
     int64_t n, float *__restrict__ s, float *__restrict__ r, float *__restrict__ o) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) s[3 * i + k] = expf(ls[3 * i + k]);
+  const float3 l = ld_f3(ls + 3 * i);
+  st_f3(s + 3 * i, expf(l.x), expf(l.y), expf(l.z));
   const float4 v = reinterpret_cast<const float4 *>(q)[i];
   const float d = fmaxf(sqrtf(((v.x * v.x + v.y * v.y) + v.z * v.z) + v.w * v.w), 1e-12f);
   reinterpret_cast<float4 *>(r)[i] = make_float4(v.x / d, v.y / d, v.z / d, v.w / d);
@@ -37,8 +37,12 @@ __global__ __launch_bounds__(kThreads) void activate_bwd_kernel(
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   if (dls != nullptr) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) dls[3 * i + k] = gs != nullptr ? gs[3 * i + k] * s[3 * i + k] : 0.0f;
+    if (gs != nullptr) {
+      const float3 g = ld_f3(gs + 3 * i), sv = ld_f3(s + 3 * i);
+      st_f3(dls + 3 * i, g.x * sv.x, g.y * sv.y, g.z * sv.z);
+    } else {
+      st_f3(dls + 3 * i, 0.0f, 0.0f, 0.0f);
+    }
   }
   if (dq != nullptr) {
     float4 g = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -47,12 +47,23 @@ __device__ __forceinline__ void load_sh(const float *__restrict__ shs,
                                         const float *__restrict__ rest, int64_t i,
                                         float (&sh)[KE * 3]) {
   if (rest != nullptr) {
-    sh[0] = shs[3 * i];
-    sh[1] = shs[3 * i + 1];
-    sh[2] = shs[3 * i + 2];
+    const float3 dc = ld_f3(shs + 3 * i);
+    sh[0] = dc.x;
+    sh[1] = dc.y;
+    sh[2] = dc.z;
+    // the rest row (3(K-1) floats) as 16-byte accesses plus the tail
     const float *rb = rest + i * ((KS - 1) * 3);
+    constexpr int nr = KE * 3 - 3;
 #pragma unroll
-    for (int j = 3; j < KE * 3; ++j) sh[j] = rb[j - 3];
+    for (int j = 0; j + 4 <= nr; j += 4) {
+      const float4 t = ld_f4u(rb + j);
+      sh[3 + j] = t.x;
+      sh[3 + j + 1] = t.y;
+      sh[3 + j + 2] = t.z;
+      sh[3 + j + 3] = t.w;
+    }
+#pragma unroll
+    for (int j = nr & ~3; j < nr; ++j) sh[3 + j] = rb[j];
     return;
   }
   const float *base = shs + i * (KS * 3);
@@ -93,13 +104,12 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const CamV cam = cam_load(intr, extr);
-  const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  const float3 x = ld_f3(xyz + 3 * i);
   // requested with the position; stored once both have arrived
   const float o_raw = opac_act != nullptr ? opac_raw[i] : 0.0f;
   float u, v, d;
   project(cam, x, nearest, extent, W, H, u, v, d);
-  uv[2 * i] = u;
-  uv[2 * i + 1] = v;
+  st_f2(uv + 2 * i, u, v);
   depth[i] = d;
   if (opac_act != nullptr) opac_act[i] = act_opacity(o_raw);
   float con[3] = {0.f, 0.f, 0.f};
@@ -107,8 +117,8 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
   float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int r = 0, t = 0;
   if (d != 0.0f) {
-    float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-    float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    float3 s = ld_f3(scale + 3 * i);
+    float4 q = ld_f4u(rot + 4 * i);
     // the SH coefficients are requested together with scale and rotation
     // (one memory round trip instead of two); almost every splat inside the
     // frustum is also rasterised
@@ -164,12 +174,15 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < 6; ++k) cov_out[6 * i + k] = cov[k];
   }
-  conic[3 * i] = con[0];
-  conic[3 * i + 1] = con[1];
-  conic[3 * i + 2] = con[2];
+  st_f3(conic + 3 * i, con[0], con[1], con[2]);
   radius[i] = r;
   tiles[i] = t;
-  for (int c = 0; c < C; ++c) feat[i * C + c] = f[c];
+  if (C == 7) {  // rgb | depth | normal: one 16-byte and one 12-byte store
+    st_f4u(feat + 7 * i, f[0], f[1], f[2], f[3]);
+    st_f3(feat + 7 * i + 4, f[4], f[5], f[6]);
+  } else {
+    for (int c = 0; c < C; ++c) feat[i * C + c] = f[c];
+  }
 }
 
 template <int KS, int KE>
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     float dsh[KE * 3];
 #pragma unroll
     for (int j = 0; j < KE * 3; ++j) dsh[j] = 0.0f;
-    const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    const float3 x = ld_f3(xyz + 3 * i);
     // inputs that do not depend on visibility are requested with the position
     const float g_depth_in = d_depth != nullptr ? d_depth[i] : 0.0f;
     const float o_raw = d_opac_raw != nullptr ? opac_raw[i] : 0.0f;
@@ -210,16 +223,16 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
     float q_norm = 1.0f;
     if (radius[i] > 0) {
       // every input of the splat is requested up front (one memory round trip)
-      float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-      float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+      float3 s = ld_f3(scale + 3 * i);
+      float4 q = ld_f4u(rot + 4 * i);
       float sh[KE * 3];
       load_sh<KS, KE>(shs, shs_rest, i, sh);
       float gf[7];
 #pragma unroll
       for (int c = 0; c < 7; ++c) gf[c] = c < C ? d_feat[i * ld_feat + c] : 0.0f;
       const float guv[2] = {d_uv[i * ld_uv], d_uv[i * ld_uv + 1]};
-      const float g3[3] = {d_conic[i * ld_conic], d_conic[i * ld_conic + 1],
-                           d_conic[i * ld_conic + 2]};
+      const float3 gc3 = ld_f3(d_conic + i * ld_conic);
+      const float g3[3] = {gc3.x, gc3.y, gc3.z};
       if (flags & kFlagRaw) {
         s = act_scale(s);
         q = act_rot(q, q_norm);
@@ -290,9 +303,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       project(cam, x, nearest, extent, W, H, u, v, dd);
       if (dd != 0.0f) project_vjp(cam, x, 0.0f, 0.0f, g_depth_in, gx, gcam);
     }
-    d_xyz[3 * i] = gx.x;
-    d_xyz[3 * i + 1] = gx.y;
-    d_xyz[3 * i + 2] = gx.z;
+    st_f3(d_xyz + 3 * i, gx.x, gx.y, gx.z);
     if (flags & kFlagRaw) {
       // through exp: ds_raw = ds * s; through normalize: (g - q (q.g)) / |q|
       // (1 / 1e-12 where the norm is clamped); through sigmoid: s (1 - s)
@@ -313,14 +324,10 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         d_opac_raw[i] = g_oact * (a * (1.0f - a));
       }
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) d_rot[4 * i + k] = gq[k];
+    st_f3(d_scale + 3 * i, gs[0], gs[1], gs[2]);
+    st_f4u(d_rot + 4 * i, gq[0], gq[1], gq[2], gq[3]);
     if (d_shs_rest != nullptr) {
-      d_shs[3 * i] = dsh[0];
-      d_shs[3 * i + 1] = dsh[1];
-      d_shs[3 * i + 2] = dsh[2];
+      st_f3(d_shs + 3 * i, dsh[0], dsh[1], dsh[2]);
       if constexpr (KS > 1) {
         // this splat's row of the rest gradient, staged for the block's
         // coalesced store below

@@ -22,6 +22,46 @@ constexpr float kJClamp = 1.3f;
 // changing any decision.
 constexpr float kPowerCut = -5.6f;
 
+// Per-splat rows ([N,2], [N,3], [N,4], [N,7] float arrays) read and written
+// as one vector access per row instead of one per float: each access costs the
+// address unit one instruction for the wave, and the preprocess kernels are
+// bound by that unit when every lane issues a dword access per value.  The
+// types carry 4-byte alignment (a row of 3 floats is 12-byte strided); gfx950
+// performs unaligned global vector accesses.
+typedef float gsr_f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float gsr_f3u __attribute__((ext_vector_type(3), aligned(4)));
+typedef float gsr_f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ float3 ld_f3(const float *p) {
+  const gsr_f3u v = *reinterpret_cast<const gsr_f3u *>(p);
+  return make_float3(v.x, v.y, v.z);
+}
+__device__ __forceinline__ float4 ld_f4u(const float *p) {
+  const gsr_f4u v = *reinterpret_cast<const gsr_f4u *>(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_f2(float *p, float a, float b) {
+  gsr_f2u v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<gsr_f2u *>(p) = v;
+}
+__device__ __forceinline__ void st_f3(float *p, float a, float b, float c) {
+  gsr_f3u v;
+  v.x = a;
+  v.y = b;
+  v.z = c;
+  *reinterpret_cast<gsr_f3u *>(p) = v;
+}
+__device__ __forceinline__ void st_f4u(float *p, float a, float b, float c, float d) {
+  gsr_f4u v;
+  v.x = a;
+  v.y = b;
+  v.z = c;
+  v.w = d;
+  *reinterpret_cast<gsr_f4u *>(p) = v;
+}
+
 // exp(x) for x <= 0 (inputs below -80 are clamped; exp(-80) ~ 1.8e-35 is far
 // below any alpha threshold).  Cody-Waite reduction x = k*ln2 + r with ln2
 // split into a 15-bit head (k*head is exact for |k| < 2^8) and a tail, then
