@@ -523,7 +523,8 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
     // the position is requested with the radius (one memory round trip); the
     // empty asm consumes it here so the compiler cannot sink the load into
     // the branch
-    const float pu = uv[2 * (int64_t)i], pv = uv[2 * (int64_t)i + 1];
+    const float2 puv = ld_f2(uv + 2 * (int64_t)i);
+    const float pu = puv.x, pv = puv.y;
     asm volatile("" : : "v"(pu), "v"(pv));
     uint2 packed = make_uint2(0u, 0u);
     if (r > 0) {

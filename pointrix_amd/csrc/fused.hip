@@ -228,9 +228,22 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
       float sh[KE * 3];
       load_sh<KS, KE>(shs, shs_rest, i, sh);
       float gf[7];
+      if (C == 7) {  // rgb | depth | normal: one 16-byte and one 12-byte access
+        const float4 a = ld_f4u(d_feat + i * ld_feat);
+        const float3 b = ld_f3(d_feat + i * ld_feat + 4);
+        gf[0] = a.x;
+        gf[1] = a.y;
+        gf[2] = a.z;
+        gf[3] = a.w;
+        gf[4] = b.x;
+        gf[5] = b.y;
+        gf[6] = b.z;
+      } else {
 #pragma unroll
-      for (int c = 0; c < 7; ++c) gf[c] = c < C ? d_feat[i * ld_feat + c] : 0.0f;
-      const float guv[2] = {d_uv[i * ld_uv], d_uv[i * ld_uv + 1]};
+        for (int c = 0; c < 7; ++c) gf[c] = c < C ? d_feat[i * ld_feat + c] : 0.0f;
+      }
+      const float2 guv2 = ld_f2(d_uv + i * ld_uv);
+      const float guv[2] = {guv2.x, guv2.y};
       const float3 gc3 = ld_f3(d_conic + i * ld_conic);
       const float g3[3] = {gc3.x, gc3.y, gc3.z};
       if (flags & kFlagRaw) {

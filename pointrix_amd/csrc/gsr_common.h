@@ -36,6 +36,10 @@ __device__ __forceinline__ float3 ld_f3(const float *p) {
   const gsr_f3u v = *reinterpret_cast<const gsr_f3u *>(p);
   return make_float3(v.x, v.y, v.z);
 }
+__device__ __forceinline__ float2 ld_f2(const float *p) {
+  const gsr_f2u v = *reinterpret_cast<const gsr_f2u *>(p);
+  return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ float4 ld_f4u(const float *p) {
   const gsr_f4u v = *reinterpret_cast<const gsr_f4u *>(p);
   return make_float4(v.x, v.y, v.z, v.w);

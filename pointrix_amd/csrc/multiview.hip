@@ -55,12 +55,23 @@ __device__ __forceinline__ void mv_load_sh(const float *__restrict__ shs,
                                            const float *__restrict__ rest, int64_t i,
                                            float (&sh)[KE * 3]) {
   if (rest != nullptr) {
-    sh[0] = shs[3 * i];
-    sh[1] = shs[3 * i + 1];
-    sh[2] = shs[3 * i + 2];
+    const float3 dc = ld_f3(shs + 3 * i);
+    sh[0] = dc.x;
+    sh[1] = dc.y;
+    sh[2] = dc.z;
+    // the rest row as 16-byte accesses plus the tail
     const float *rb = rest + i * ((KS - 1) * 3);
+    constexpr int nr = KE * 3 - 3;
 #pragma unroll
-    for (int j = 3; j < KE * 3; ++j) sh[j] = rb[j - 3];
+    for (int j = 0; j + 4 <= nr; j += 4) {
+      const float4 t = ld_f4u(rb + j);
+      sh[3 + j] = t.x;
+      sh[3 + j + 1] = t.y;
+      sh[3 + j + 2] = t.z;
+      sh[3 + j + 3] = t.w;
+    }
+#pragma unroll
+    for (int j = nr & ~3; j < nr; ++j) sh[3 + j] = rb[j];
   } else {
     const float *b = shs + i * (KS * 3);
 #pragma unroll
@@ -89,9 +100,9 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_fwd_kernel(
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float3 x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-  const float3 s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-  const float4 q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+  const float3 x = ld_f3(xyz + 3 * i);
+  const float3 s = ld_f3(scale + 3 * i);
+  const float4 q = ld_f4u(rot + 4 * i);
   float cov[6];
   covariance3d(s, q, cov);
   float sh[KE * 3];
@@ -158,9 +169,7 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_fwd_kernel(
     uv[2 * j] = u;
     uv[2 * j + 1] = vv;
     depth[j] = d;
-    conic[3 * j] = con[0];
-    conic[3 * j + 1] = con[1];
-    conic[3 * j + 2] = con[2];
+    st_f3(conic + 3 * j, con[0], con[1], con[2]);
     for (int c = 0; c < C; ++c) feat[j * C + c] = f[c];
   }
 }
@@ -189,9 +198,9 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
   float3 x = make_float3(0.f, 0.f, 0.f), s = make_float3(1.f, 1.f, 1.f);
   float4 q = make_float4(1.f, 0.f, 0.f, 0.f);
   if (active) {
-    x = make_float3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
-    s = make_float3(scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]);
-    q = make_float4(rot[4 * i], rot[4 * i + 1], rot[4 * i + 2], rot[4 * i + 3]);
+    x = ld_f3(xyz + 3 * i);
+    s = ld_f3(scale + 3 * i);
+    q = ld_f4u(rot + 4 * i);
   }
   float cov[6];
   covariance3d(s, q, cov);
@@ -306,9 +315,7 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
   if (active) {
     float gs[3], gq2[4];
     covariance3d_vjp(s, q, gS, gs, gq2);
-    d_xyz[3 * i] = gx.x;
-    d_xyz[3 * i + 1] = gx.y;
-    d_xyz[3 * i + 2] = gx.z;
+    st_f3(d_xyz + 3 * i, gx.x, gx.y, gx.z);
     d_opacity[i] = gop;
 #pragma unroll
     for (int k = 0; k < 3; ++k) d_scale[3 * i + k] = gs[k];
@@ -317,9 +324,7 @@ __global__ __launch_bounds__(kThreads) void mv_preprocess_bwd_kernel(
   }
   if (d_shs_rest != nullptr) {
     if (active) {
-      d_shs[3 * i] = dsh[0];
-      d_shs[3 * i + 1] = dsh[1];
-      d_shs[3 * i + 2] = dsh[2];
+      st_f3(d_shs + 3 * i, dsh[0], dsh[1], dsh[2]);
     }
     if constexpr (KS > 1) {
       // stage the rows, then store the block's contiguous range of
