@@ -37,6 +37,26 @@ struct TileView {
 
 constexpr int kMaxTileWaves = 4;  // 256-pixel tiles, one wave per 64 pixels
 
+// A splat's feature row [C] into batch column dst[c * BD]; rows of 7 (rgb |
+// depth | normal) as one 16-byte and one 12-byte access.
+template <int C, int BD>
+__device__ __forceinline__ void load_feat_row(const float *__restrict__ row, float *dst) {
+  if constexpr (C == 7) {
+    const float4 a = ld_f4u(row);
+    const float3 b = ld_f3(row + 4);
+    dst[0] = a.x;
+    dst[BD] = a.y;
+    dst[2 * BD] = a.z;
+    dst[3 * BD] = a.w;
+    dst[4 * BD] = b.x;
+    dst[5 * BD] = b.y;
+    dst[6 * BD] = b.z;
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; ++c) dst[c * BD] = row[c];
+  }
+}
+
 // LDS of one tile: a batch of BD splats (BD <= TW*TH; the "batch depth"), and
 // for the forward the per-wave lists of batch slots that can reach the wave's
 // pixels (slot indices in depth order) with their per-loading-wave counts.
@@ -96,16 +116,15 @@ __device__ __forceinline__ void blend_fwd_tile(
     unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < end) {
       const int g = ids[j];
-      const float u = uv[2 * (int64_t)g], v = uv[2 * (int64_t)g + 1];
-      const float ca = conic[3 * (int64_t)g], cb = conic[3 * (int64_t)g + 1],
-                  cc = conic[3 * (int64_t)g + 2];
+      const float2 uv2 = ld_f2(uv + 2 * (int64_t)g);
+      const float3 co3 = ld_f3(conic + 3 * (int64_t)g);
+      const float u = uv2.x, v = uv2.y, ca = co3.x, cb = co3.y, cc = co3.z;
       const float o = opacity[g - tv.op_offset];
       SplatLds sp;
       sp.xy = make_float2(u, v);
       sp.co = make_float4(-0.5f * ca, -cb, -0.5f * cc, o);
       sm.sp[threadIdx.x] = sp;
-#pragma unroll
-      for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
+      load_feat_row<C, BD>(feat + (int64_t)g * C, sm.f + threadIdx.x);
       bool reach;
       const TightSplat ts = tight_setup(u, v, ca, cb, cc, o, reach);
       if (reach) {
@@ -320,17 +339,16 @@ __device__ __forceinline__ void blend_bwd_tile2(
     unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < bend) {
       const int g = ids[j];
-      const float u = uv[2 * (int64_t)g], v = uv[2 * (int64_t)g + 1];
-      const float ca = conic[3 * (int64_t)g], cb = conic[3 * (int64_t)g + 1],
-                  cc = conic[3 * (int64_t)g + 2];
+      const float2 uv2 = ld_f2(uv + 2 * (int64_t)g);
+      const float3 co3 = ld_f3(conic + 3 * (int64_t)g);
+      const float u = uv2.x, v = uv2.y, ca = co3.x, cb = co3.y, cc = co3.z;
       const float o = opacity[g - tv.op_offset];
       SplatLds sp;
       sp.xy = make_float2(u, v);
       sp.co = make_float4(-0.5f * ca, -cb, -0.5f * cc, o);
       sm.sp[threadIdx.x] = sp;
       sm.id[threadIdx.x] = g;
-#pragma unroll
-      for (int c = 0; c < C; ++c) sm.f[c * BD + threadIdx.x] = feat[(int64_t)g * C + c];
+      load_feat_row<C, BD>(feat + (int64_t)g * C, sm.f + threadIdx.x);
       // the forward's reachability test per strip: a splat no pixel of the
       // strip can see at alpha >= kAlphaMin was neither a contributor nor a
       // stopping splat there, so the strip's walk leaves it out
