@@ -611,50 +611,60 @@ struct StoreCompact {
 // Row/column of instance k within the w-wide rectangle: q = trunc((k + 0.5)
 // / w) in float is exact while k < 2^22 / w (k is never within 0.5/w of a
 // multiple of w), far beyond the largest tile grid.
-__global__ __launch_bounds__(kThreads) void emit_kernel(
-    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
-    const uint2 *__restrict__ rects, const uint32_t *nv_ptr, uint32_t cap, Grid g,
-    uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
-  __shared__ uint32_t s_flag[kThreads];
-  const uint32_t nv = count_of(nv_ptr, cap);
-  const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
-  if (wave_base >= nv) return;  // uniform per wave
-  const int lane = lane_id();
-  uint32_t *flag = s_flag + (threadIdx.x & ~63u);
-  const uint32_t s = wave_base + lane;
-  const bool mine = s < nv;
-  int id = 0, toff = 0;
-  uint32_t xy = 0, wgx = 1, off = 0xFFFFFFFFu, end = 0;
-  float inv_w = 1.0f;
-  if (mine) {
-    id = sorted_ids[s];
-    const ItemGrid ig = item_grid(g, (uint32_t)id);
-    toff = ig.toff;
-    const uint2 r = rects[id];
-    xy = r.x;  // x0 | y0 << 16
+// The wave's 64 depth-sorted splats, one per lane (as the count kernel
+// packed their rectangles), and the instance range [r0, r1) they own (r1
+// truncated to the instance capacity: a caller that sized the instance
+// buffers below the device count -- graph replays -- gets an in-bounds prefix).
+struct EmitWave {
+  int id, toff;
+  uint32_t xy, wgx, off, end;
+  float inv_w;
+  uint32_t r0, r1;
+};
+
+__device__ __forceinline__ EmitWave emit_wave_load(const int32_t *__restrict__ sorted_ids,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const uint2 *__restrict__ rects,
+                                                   uint32_t nv, const Grid &g,
+                                                   uint32_t wave_base, uint32_t cap_inst) {
+  EmitWave e{0, 0, 0u, 1u, 0xFFFFFFFFu, 0u, 1.0f, 0u, 0u};
+  const uint32_t s = wave_base + (uint32_t)lane_id();
+  if (s < nv) {
+    e.id = sorted_ids[s];
+    const ItemGrid ig = item_grid(g, (uint32_t)e.id);
+    e.toff = ig.toff;
+    const uint2 r = rects[e.id];
+    e.xy = r.x;  // x0 | y0 << 16
     const uint32_t rw = r.y & 0xFFFFu, rh = r.y >> 16;
     const uint32_t w = rw > 1u ? rw : 1u;
-    wgx = w | ((uint32_t)ig.gx << 16);  // the tile grid is < 65536 per side
-    inv_w = 1.0f / (float)w;
-    off = offsets[s];
-    end = off + rw * rh;
+    e.wgx = w | ((uint32_t)ig.gx << 16);  // the tile grid is < 65536 per side
+    e.inv_w = 1.0f / (float)w;
+    e.off = offsets[s];
+    e.end = e.off + rw * rh;
   }
   const int nsplat = min(64, (int)(nv - wave_base));
-  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)off, 0);
-  const uint32_t r1 = min((uint32_t)__builtin_amdgcn_readlane((int)end, nsplat - 1), cap_inst);
+  e.r0 = (uint32_t)__builtin_amdgcn_readlane((int)e.off, 0);
+  e.r1 = min((uint32_t)__builtin_amdgcn_readlane((int)e.end, nsplat - 1), cap_inst);
+  return e;
+}
+
+// Instances [cb, ce) of the wave's range, 64 consecutive ones per round.
+__device__ __forceinline__ void emit_rounds(const EmitWave &e, uint32_t cb, uint32_t ce,
+                                            uint32_t *flag, uint32_t *__restrict__ tkeys,
+                                            int32_t *__restrict__ tvals) {
+  const int lane = lane_id();
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  // capacity guard: a caller that sized the instance buffers below the device
-  // count (graph replays) gets a truncated, in-bounds emission
-  for (uint32_t c0 = r0; c0 < r1; c0 += 64) {
-    const int j0 = 63 - __clzll(__ballot(off <= c0));  // splat holding instance c0
-    const uint32_t jend = min((uint32_t)__builtin_amdgcn_readlane((int)end, j0), r1);
+  for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
+    const int j0 = 63 - __clzll(__ballot(e.off <= c0));  // splat holding instance c0
+    const uint32_t jend = min((uint32_t)__builtin_amdgcn_readlane((int)e.end, j0), ce);
     if (jend >= c0 + 64u) {
-      // whole chunks inside splat j0 (uniform): no owner search
-      const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)off, j0);
-      const uint32_t jxy = (uint32_t)__builtin_amdgcn_readlane((int)xy, j0);
-      const uint32_t jwgx = (uint32_t)__builtin_amdgcn_readlane((int)wgx, j0);
-      const int jtoff = __builtin_amdgcn_readlane(toff, j0), jid = __builtin_amdgcn_readlane(id, j0);
-      const float jinv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inv_w), j0));
+      // whole rounds inside splat j0 (uniform): no owner search
+      const uint32_t joff = (uint32_t)__builtin_amdgcn_readlane((int)e.off, j0);
+      const uint32_t jxy = (uint32_t)__builtin_amdgcn_readlane((int)e.xy, j0);
+      const uint32_t jwgx = (uint32_t)__builtin_amdgcn_readlane((int)e.wgx, j0);
+      const int jtoff = __builtin_amdgcn_readlane(e.toff, j0);
+      const int jid = __builtin_amdgcn_readlane(e.id, j0);
+      const float jinv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.inv_w), j0));
       const int w = (int)(jwgx & 0xFFFFu), gx = (int)(jwgx >> 16);
       const int x0 = (int)(jxy & 0xFFFFu), y0 = (int)(jxy >> 16);
       for (; c0 + 64u <= jend; c0 += 64) {
@@ -665,23 +675,23 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
         tkeys[q] = (uint32_t)(jtoff + (y0 + row) * gx + (x0 + col));
         tvals[q] = jid;
       }
-      c0 -= 64;  // the loop increment moves to the first chunk not yet written
+      c0 -= 64;  // the loop increment moves to the first round not yet written
       continue;
     }
     flag[lane] = 0u;
     __builtin_amdgcn_wave_barrier();
-    const uint32_t p = off - c0;  // chunk position of this splat's first instance
+    const uint32_t p = e.off - c0;  // round position of this splat's first instance
     if (p >= 1u && p < 64u) flag[p] = 1u;
     __builtin_amdgcn_wave_barrier();
     const uint64_t starts = __ballot(flag[lane] != 0u);
     __builtin_amdgcn_wave_barrier();
     const int o = j0 + (int)__popcll(starts & upto);
-    const uint32_t ooff = (uint32_t)shfl_i((int)off, o);
-    const uint32_t oxy = (uint32_t)shfl_i((int)xy, o), owgx = (uint32_t)shfl_i((int)wgx, o);
-    const int otoff = shfl_i(toff, o), oid = shfl_i(id, o);
-    const float oinv = shfl_f(inv_w, o);
+    const uint32_t ooff = (uint32_t)shfl_i((int)e.off, o);
+    const uint32_t oxy = (uint32_t)shfl_i((int)e.xy, o), owgx = (uint32_t)shfl_i((int)e.wgx, o);
+    const int otoff = shfl_i(e.toff, o), oid = shfl_i(e.id, o);
+    const float oinv = shfl_f(e.inv_w, o);
     const uint32_t q = c0 + (uint32_t)lane;
-    if (q < r1) {
+    if (q < ce) {
       const int k = (int)(q - ooff);
       const int w = (int)(owgx & 0xFFFFu), gx = (int)(owgx >> 16);
       const int row = (int)(((float)k + 0.5f) * oinv);
@@ -690,6 +700,56 @@ __global__ __launch_bounds__(kThreads) void emit_kernel(
       tvals[q] = oid;
     }
   }
+}
+
+// A wave writes at most kEmitSpan instances of its range; the rest (the waves
+// of large near splats) is queued as items of kEmitChunk instances that
+// emit_rest_kernel spreads over the chip, one atomic per such wave.
+constexpr uint32_t kEmitSpan = 8192;
+constexpr uint32_t kEmitChunk = 4096;
+// items[] bound: a wave queues items only beyond kEmitSpan instances, so at
+// most M / kEmitSpan waves queue, each at most (its instances) / kEmitChunk + 1
+inline uint64_t emit_items_cap(uint64_t m) { return m / kEmitChunk + m / kEmitSpan + 1; }
+
+__global__ __launch_bounds__(kThreads) void emit_kernel(
+    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
+    const uint2 *__restrict__ rects, const uint32_t *nv_ptr, uint32_t cap, Grid g,
+    uint32_t cap_inst, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals,
+    uint2 *__restrict__ items, uint32_t *__restrict__ n_items, uint32_t items_cap) {
+  __shared__ uint32_t s_flag[kThreads];
+  const uint32_t nv = count_of(nv_ptr, cap);
+  const uint32_t wave_base = (blockIdx.x * kThreads + threadIdx.x) & ~63u;
+  if (wave_base >= nv) return;  // uniform per wave
+  const EmitWave e = emit_wave_load(sorted_ids, offsets, rects, nv, g, wave_base, cap_inst);
+  // r1 < r0 when the capacity truncates before this wave's range: nothing to do
+  const uint32_t len = e.r1 > e.r0 ? e.r1 - e.r0 : 0u;
+  const uint32_t ce = e.r0 + min(len, kEmitSpan);
+  if (ce < e.r1) {  // uniform: queue the rest
+    const uint32_t nit = (e.r1 - ce + kEmitChunk - 1) / kEmitChunk;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(n_items, nit);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    for (uint32_t t = (uint32_t)lane_id(); t < nit; t += 64)
+      if (base + t < items_cap) items[base + t] = make_uint2(wave_base, ce + t * kEmitChunk);
+  }
+  emit_rounds(e, e.r0, ce, s_flag + (threadIdx.x & ~63u), tkeys, tvals);
+}
+
+// The queued rest: wave v takes item v (a wave's 64 splats and the first
+// instance of a kEmitChunk range of theirs).
+__global__ __launch_bounds__(kThreads) void emit_rest_kernel(
+    const int32_t *__restrict__ sorted_ids, const uint32_t *__restrict__ offsets,
+    const uint2 *__restrict__ rects, const uint32_t *nv_ptr, uint32_t cap, Grid g,
+    uint32_t cap_inst, const uint2 *__restrict__ items, const uint32_t *__restrict__ n_items,
+    uint32_t items_cap, uint32_t *__restrict__ tkeys, int32_t *__restrict__ tvals) {
+  __shared__ uint32_t s_flag[kThreads];
+  const uint32_t v = (blockIdx.x * kThreads + threadIdx.x) >> 6;
+  if (v >= min(*n_items, items_cap)) return;  // uniform per wave
+  const uint2 item = items[v];
+  const uint32_t nv = count_of(nv_ptr, cap);
+  const EmitWave e = emit_wave_load(sorted_ids, offsets, rects, nv, g, item.x, cap_inst);
+  const uint32_t cb = item.y, ce = min(item.y + kEmitChunk, e.r1);
+  emit_rounds(e, cb, ce, s_flag + (threadIdx.x & ~63u), tkeys, tvals);
 }
 
 // Tight emission in splat index order, as the tight count walked them: the
@@ -768,6 +828,16 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(int32_t *__restrict__ p,
                                                         int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Zero-fill of p[0, n) and of q[0, nq) in one launch.
+__global__ __launch_bounds__(kThreads) void zero2_kernel(int32_t *__restrict__ p, int64_t n,
+                                                         uint32_t *__restrict__ q, int64_t nq) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+  else if (i - n < nq)
+    q[i - n] = 0u;
 }
 
 // tile_range[t] = [first, last + 1) of tile t in the sorted keys: a key that
@@ -927,7 +997,10 @@ extern "C" {
 
 size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n > 0 ? n : 1)); }
 
-size_t gsr_sort_ws_inst_bytes(int64_t m) { return sortbufs_bytes((uint64_t)(m > 0 ? m : 1)); }
+size_t gsr_sort_ws_inst_bytes(int64_t m) {
+  const uint64_t c = (uint64_t)(m > 0 ? m : 1);
+  return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8);
+}
 
 }  // extern "C"
 
@@ -990,23 +1063,36 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
 int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid &gr, int T,
                    void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
                    int32_t *tile_range, hipStream_t s) {
-  fill_kernel<<<gsr_blocks(2 * (int64_t)T, kThreads), kThreads, 0, s>>>(tile_range, 2 * (int64_t)T,
-                                                                        0);
-  if (m == 0 || n == 0) return gsr_launch_status("sort_emit");
+  if (m == 0 || n == 0) {
+    fill_kernel<<<gsr_blocks(2 * (int64_t)T, kThreads), kThreads, 0, s>>>(tile_range,
+                                                                          2 * (int64_t)T, 0);
+    return gsr_launch_status("sort_emit");
+  }
   GaussWs g;
   carve_gauss(ws_gauss, (uint64_t)n, g);
   SortBufs ib;
-  carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
+  char *p = carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
+  uint32_t *n_items;
+  uint2 *items;
+  const uint64_t items_cap = emit_items_cap((uint64_t)m);
+  p = carve(p, 64, (void **)&n_items);
+  carve(p, items_cap * 8, (void **)&items);
+  // tile_range and the emission's queue counter start at zero
+  zero2_kernel<<<gsr_blocks(2 * (int64_t)T + 1, kThreads), kThreads, 0, s>>>(
+      tile_range, 2 * (int64_t)T, n_items, 1);
   const int nbits = tile_bits(T);
   const int passes = radix_passes(nbits);
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;
-  if (gr.conic == nullptr)
-    emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(g.sb.valA, g.offs, g.rects,
-                                                             g.scalars, (uint32_t)n, gr,
-                                                             (uint32_t)m, ib.keyA, v_first);
-  else
+  if (gr.conic == nullptr) {
+    emit_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(
+        g.sb.valA, g.offs, g.rects, g.scalars, (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first,
+        items, n_items, (uint32_t)items_cap);
+    emit_rest_kernel<<<gsr_blocks((int64_t)items_cap * 64, kThreads), kThreads, 0, s>>>(
+        g.sb.valA, g.offs, g.rects, g.scalars, (uint32_t)n, gr, (uint32_t)m, items, n_items,
+        (uint32_t)items_cap, ib.keyA, v_first);
+  } else
     emit_tight_kernel<<<gsr_blocks(n, kThreads), kThreads, 0, s>>>(
         uv, radius, g.cnt, reinterpret_cast<const uint32_t *>(g.rects), g.offs, g.masks,
         (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
