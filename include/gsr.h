@@ -93,7 +93,9 @@ int gsr_compute_sh_bwd(const float *shs, int64_t s_n, int64_t s_c, int64_t s_k,
  * log scale, rot an unnormalised quaternion, and the kernel applies exp and
  * F.normalize (GaussianPointCloud's activations, gaussian_points.py:30-35,
  * 70-81); with flag 4, opacity_raw[N] (logits) -> opacity_act[N] = sigmoid
- * (both NULL, or both set). */
+ * (both NULL, or both set).  flags 8 = plain colour: rgb is the SH sum
+ * itself, without the +0.5 and the clamp (MsplatNormalRender,
+ * examples/supervise/renderer.py:41-42). */
 int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, const float *shs,
                        const float *shs_rest, int64_t n, int K, int degree, const float *intr, const float *extr,
                        const float *cam_center, int W, int H, float nearest, float extent,

@@ -41,6 +41,9 @@ def lib():
         L.ref_blend.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P]
         L.ref_preprocess.argtypes = [P, P, P, P, I64, I, I, P, P, P, I, I, F, F, I, P, P, P, P, P,
                                      P]
+        L.ref_blend_bwd.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P, P, I64, I64, P,
+                                    P, P, P, P]
+        L.ref_preprocess_bwd.argtypes = [P, P, P, P, I64, I, I, P, P, P, I, I, F, F, I] + [P] * 11
         _lib = L
     return _lib
 
@@ -176,3 +179,126 @@ def preprocess(xyz, scale, rot, shs, intr, extr, center, W, H, degree=-1, flags=
                          _p(K), _p(E), _p(cen), W, H, nearest, extent, flags, _p(uv), _p(d),
                          _p(conic), _p(radius), _p(tiles), _p(feat))
     return uv, d, conic, radius, tiles, feat
+
+
+def alpha_blending_bwd(uv, conic, opacity, feature, ids, tile_range, bg, W, H, T_final,
+                       n_contrib, dout, dT=None, tile_w=16, tile_h=16):
+    """Backward of alpha_blending from dout [C,H,W] (and dT [H,W]) ->
+    (d_uv [n,2], d_conic [n,3], d_opacity [n], d_feature [n,C])."""
+    u, c = _c(uv, np.float32), _c(conic, np.float32)
+    o = _c(np.asarray(opacity).reshape(-1), np.float32)
+    f = _c(feature, np.float32)
+    i = _c(ids, np.int32)
+    r = _c(tile_range, np.int32)
+    T, nc, g = _c(T_final, np.float32), _c(n_contrib, np.int32), _c(dout, np.float32)
+    gT = None if dT is None else _c(dT, np.float32)
+    n, C = f.shape
+    m = i.shape[0]
+    rec = np.empty((max(m, 1), 6 + C), np.float32)
+    d_uv = np.zeros((n, 2), np.float32)
+    d_c = np.zeros((n, 3), np.float32)
+    d_o = np.zeros(n, np.float32)
+    d_f = np.zeros((n, C), np.float32)
+    lib().ref_blend_bwd(_p(u), _p(c), _p(o), _p(f), C, _p(i), _p(r), bg, W, H, tile_w, tile_h,
+                        _p(T), _p(nc), _p(g), None if gT is None else _p(gT), n, m, _p(rec),
+                        _p(d_uv), _p(d_c), _p(d_o), _p(d_f))
+    return d_uv, d_c, d_o, d_f
+
+
+def preprocess_bwd(xyz, scale, rot, shs, intr, extr, center, W, H, radius, d_uv, d_conic,
+                   d_feat, degree=-1, flags=0, nearest=0.2, extent=1.3):
+    """Backward of preprocess -> (d_xyz [n,3], d_scale [n,3], d_rot [n,4],
+    d_shs [n,K,3], d_intr [4], d_extr [3,4], d_center [3])."""
+    x, sc, q = _c(xyz, np.float32), _c(scale, np.float32), _c(rot, np.float32)
+    sh = _c(shs, np.float32)
+    K = _c(np.asarray(intr).reshape(-1)[:4], np.float32)
+    E = _c(np.asarray(extr)[:3, :4], np.float32)
+    cen = _c(np.asarray(center).reshape(-1)[:3], np.float32)
+    rad = _c(np.asarray(radius).reshape(-1), np.int32)
+    n, Kc = x.shape[0], sh.shape[1]
+    gu, gc, gf = _c(d_uv, np.float32), _c(d_conic, np.float32), _c(d_feat, np.float32)
+    d_x = np.zeros((n, 3), np.float32)
+    d_s = np.zeros((n, 3), np.float32)
+    d_q = np.zeros((n, 4), np.float32)
+    d_sh = np.zeros((n, Kc, 3), np.float32)
+    d_k = np.zeros(4, np.float32)
+    d_e = np.zeros((3, 4), np.float32)
+    d_c = np.zeros(3, np.float32)
+    lib().ref_preprocess_bwd(_p(x), _p(sc), _p(q), _p(sh), n, Kc,
+                             -1 if degree is None else degree, _p(K), _p(E), _p(cen), W, H,
+                             nearest, extent, flags, _p(rad), _p(gu), _p(gc), _p(gf), _p(d_x),
+                             _p(d_s), _p(d_q), _p(d_sh), _p(d_k), _p(d_e), _p(d_c))
+    return d_x, d_s, d_q, d_sh, d_k, d_e, d_c
+
+
+def normal_planes(raw):
+    """(n / |n| + 1) / 2 of a [3,H,W] blended normal (examples/supervise/
+    renderer.py:89-96), in float32."""
+    n = np.asarray(raw, np.float32)
+    r = np.sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])
+    return ((n / r + np.float32(1.0)) / np.float32(2.0)).astype(np.float32)
+
+
+def normal_planes_bwd(raw, g):
+    """Gradient of normal_planes: d n = (g - n (n.g) / |n|^2) / (2 |n|)."""
+    n = np.asarray(raw, np.float32)
+    g = np.asarray(g, np.float32)
+    r2 = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]
+    r = np.sqrt(r2)
+    ng = (n[0] * g[0] + n[1] * g[1]) + n[2] * g[2]
+    return ((g - n * (ng / r2)) / (np.float32(2.0) * r)).astype(np.float32)
+
+
+def render_step(inputs, camera, degree=3, flags=0, bg=0.0, image_grad=None, keep=False):
+    """One render_iter forward and backward on the host, FP32 throughout:
+    preprocess, binning, blending, then the blend backward and the
+    preprocess backward.
+
+    inputs: activated splat tensors as numpy arrays -- position [N,3],
+    scaling [N,3], rotation [N,4] (unit wxyz), opacity [N] or [N,1], shs
+    [N,K,3].  camera: (W, H, intr [4], extr [3,4] or [4,4], centre [3]).
+    flags: 1 depth channel, 2 camera-space normal channel, 8 plain colour.
+    image_grad(planes) -> (loss, grads): planes {"rgb", "depth"?, "normal"?}
+    as the renderer returns them (the normal group mapped to [0,1]), grads
+    the same keys.  Returns {"loss", "planes", "grads": {position, scaling,
+    rotation, opacity, shs, intr, extr, center}, and with ``keep`` the
+    forward intermediates}."""
+    W, H, K, E, cen = camera
+    x = _c(inputs["position"], np.float32)
+    s = _c(inputs["scaling"], np.float32)
+    q = _c(inputs["rotation"], np.float32)
+    op = _c(np.asarray(inputs["opacity"]).reshape(-1), np.float32)
+    sh = _c(inputs["shs"], np.float32)
+    uv, d, con, rad, tl, feat = preprocess(x, s, q, sh, K, E, cen, W, H, degree, flags)
+    ids, rng = sort_gaussian(uv, d, W, H, rad)
+    out, T, nc = alpha_blending(uv, con, op, feat, ids, rng, bg, W, H)
+    names = [("rgb", 3)] + ([("depth", 1)] if flags & 1 else []) + \
+        ([("normal", 3)] if flags & 2 else [])
+    planes, start = {}, 0
+    for name, c in names:
+        planes[name] = out[start:start + c]
+        start += c
+    if "normal" in planes:
+        planes["normal"] = normal_planes(planes["normal"])
+    res = {"planes": planes, "T_final": T, "n_contrib": nc}
+    if keep:
+        res.update(uv=uv, depth=d, conic=con, radius=rad, tiles=tl, features=feat, ids=ids,
+                   tile_range=rng, rendered=out)
+    if image_grad is None:
+        return res
+    loss, gp = image_grad(planes)
+    dout = np.zeros_like(out)
+    start = 0
+    for name, c in names:
+        g = np.asarray(gp[name], np.float32)
+        if name == "normal":
+            g = normal_planes_bwd(out[start:start + c], g)
+        dout[start:start + c] = g
+        start += c
+    du, dc, do, df = alpha_blending_bwd(uv, con, op, feat, ids, rng, bg, W, H, T, nc, dout)
+    gx, gs, gq, gsh, gk, ge, gcen = preprocess_bwd(x, s, q, sh, K, E, cen, W, H, rad, du, dc, df,
+                                                  degree, flags)
+    res["loss"] = loss
+    res["grads"] = {"position": gx, "scaling": gs, "rotation": gq, "opacity": do, "shs": gsh,
+                    "intr": gk, "extr": ge, "center": gcen, "uv": du}
+    return res

@@ -164,6 +164,7 @@ class _BatchedFn(torch.autograd.Function):
                      ctypes.c_void_p(m_host.data_ptr()), ctypes.c_void_p(ev.cuda_event), st)
                 ev.synchronize()
                 m = int(m_host[0])
+                meta["count"] = m
             else:
                 call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V,
                      ptr(vi), lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev), st)
@@ -269,6 +270,9 @@ class BatchedRender:
         # all views in self.intermediates
         self.keep_intermediates = False
         self.intermediates = None
+        # exact instance count of the last forward, as read on the host to
+        # size its buffers (None in capacity mode, where no read is made)
+        self.last_instances = None
 
     @property
     def flags(self):
@@ -332,11 +336,16 @@ class BatchedRender:
         self.intermediates = meta["keep"]
         out, T, radius, nc = _BatchedFn.apply(position, scaling, rotation, dc, rest, opacity, vf,
                                               ndc, meta)
+        self.last_instances = meta.get("count")
         return out, T, radius, ndc, lay
 
     def instances(self) -> int:
         """Device instance count of the last forward (host read)."""
         return int(self._m_dev.item()) if self._m_dev is not None else 0
+
+    def count_tensor(self):
+        """The device int32[1] the instance count of each forward is written to."""
+        return self._m_dev
 
 
 class GraphedStep(CapturedStep):

@@ -24,6 +24,9 @@ constexpr int kFlagNormal = 2;
 // normalize and sigmoid (GaussianPointCloud's activations,
 // gaussian_points.py:30-35,70-81) and return gradients w.r.t. the raw values
 constexpr int kFlagRaw = 4;
+// colour = the SH sum itself: no +0.5 offset, no clamp (the normal-supervision
+// renderer of examples/supervise/renderer.py:41-42)
+constexpr int kFlagPlain = 8;
 
 __device__ __forceinline__ float3 act_scale(float3 s) {
   return make_float3(expf(s.x), expf(s.y), expf(s.z));
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_fwd_kernel(
         float acc = sh[c] * Y[0];
 #pragma unroll
         for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
-        f[c] = fmaxf(acc + 0.5f, 0.0f);
+        f[c] = (flags & kFlagPlain) ? acc : fmaxf(acc + 0.5f, 0.0f);
       }
       int o = 3;
       if (flags & kFlagDepth) f[o++] = d;
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_bwd_kernel(
         float acc = sh[c] * Y[0];
 #pragma unroll
         for (int k = 1; k < KE; ++k) acc = acc + sh[3 * k + c] * Y[k];
-        const float g = (acc + 0.5f >= 0.0f) ? gf[c] : 0.0f;
+        const float g = ((flags & kFlagPlain) || acc + 0.5f >= 0.0f) ? gf[c] : 0.0f;
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
           dsh[3 * k + c] = g * Y[k];
@@ -474,7 +477,7 @@ int gsr_preprocess_fwd(const float *xyz, const float *scale, const float *rot, c
                        int32_t *radius, int32_t *tiles, float *feat, const float *opacity_raw,
                        float *opacity_act, void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
-                  (flags & ~7) == 0 && ((opacity_raw == nullptr) == (opacity_act == nullptr)) &&
+                  (flags & ~15) == 0 && ((opacity_raw == nullptr) == (opacity_act == nullptr)) &&
                   (opacity_raw == nullptr || (flags & kFlagRaw)),
               "preprocess_fwd: bad arguments (K=%d flags=%d)", K, flags);
   if (n == 0) return GSR_OK;
@@ -508,7 +511,7 @@ int gsr_preprocess_bwd(const float *xyz, const float *scale, const float *rot, c
                        const float *d_opacity_act, int ld_opacity_act, float *d_opacity_raw,
                        void *stream) {
   GSR_REQUIRE(n >= 0 && W > 0 && H > 0 && (K == 1 || K == 4 || K == 9 || K == 16) &&
-                  (flags & ~7) == 0 &&
+                  (flags & ~15) == 0 &&
                   ((opacity_raw == nullptr) == (d_opacity_raw == nullptr)) &&
                   ((d_opacity_raw == nullptr) || (d_opacity_act != nullptr && ld_opacity_act >= 1)) &&
                   (opacity_raw == nullptr || (flags & kFlagRaw)),

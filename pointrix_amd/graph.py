@@ -14,7 +14,8 @@ at the new size and replayed, so every result the caller sees is complete.
 
 The object that sizes the buffers (the "sizer": a renderer's sort workspace,
 or the batched renderer) exposes ``capacity`` (None = exact, with a host
-read) and ``instances()`` (the device count of the last forward).
+read), ``instances()`` (host read of the device count of the last forward)
+and ``count_tensor()`` (that device count, an int32[1] tensor).
 """
 from __future__ import annotations
 
@@ -25,11 +26,21 @@ import torch
 
 class CapturedStep:
     """``step_fn() -> loss`` runs forward + loss + ``backward()`` into the
-    ``.grad`` of ``leaves`` ({name: tensor}).  Gradients are zeroed inside the
-    graph.  The graph writes into the gradient tensors that existed at
-    capture; they are held here and bound back to the leaves' ``.grad`` on
-    every replay, so eager steps in between (which may replace ``.grad``)
-    never leave the graph writing into freed memory."""
+    ``.grad`` of ``leaves`` ({name: tensor}).
+
+    The gradients are not zeroed and accumulated: every leaf's ``.grad`` is
+    None when the captured step starts, so the backward writes fresh gradient
+    buffers (from the graph's memory pool) instead of reading, adding and
+    writing back zero-filled ones.  They are held here and bound back to the
+    leaves' ``.grad`` on every replay, so eager steps in between (which may
+    replace ``.grad``) never leave the graph writing into freed memory.
+
+    The graph also keeps the largest instance count it has seen on the device
+    (one small kernel).  ``replay(check=True)`` reads it after the replay (a
+    host wait) and re-captures when it outgrew the capacity;
+    ``replay(check=False)`` does not wait, and ``overflowed()`` tells later
+    whether any unchecked replay since the last check overflowed (its
+    results are then incomplete and the step has been captured again)."""
 
     def __init__(self, step_fn, leaves: dict, sizer, headroom=1.25, min_capacity=1024):
         self.step_fn, self.leaves, self.sizer = step_fn, leaves, sizer
@@ -38,16 +49,22 @@ class CapturedStep:
         self.loss = None
         self.grads = {}
         self.captures = 0
+        self.max_count = None
 
     def _step(self):
         for p in self.leaves.values():
-            if p.grad is not None:
-                p.grad.zero_()
-        return self.step_fn()
+            p.grad = None
+        loss = self.step_fn()
+        m = self.sizer.count_tensor()
+        if m is not None:
+            torch.maximum(self.max_count, m, out=self.max_count)
+        return loss
 
     def capture(self):
         sz = self.sizer
         sz.capacity = None
+        dev = next(iter(self.leaves.values())).device
+        self.max_count = torch.zeros(1, dtype=torch.int32, device=dev)
         self._step()                                   # eager: exact instance count
         torch.cuda.synchronize()
         m = sz.instances()
@@ -62,19 +79,33 @@ class CapturedStep:
         with torch.cuda.graph(self.graph):
             self.loss = self._step()
         self.grads = {k: p.grad for k, p in self.leaves.items()}
+        self.max_count.zero_()
         self.captures += 1
         return self
 
-    def replay(self) -> bool:
-        """Run the captured step.  Returns False when the instance count had
-        outgrown the capacity (the step was then captured again and replayed,
-        so the results are complete either way)."""
+    def overflowed(self) -> bool:
+        """Host read of the largest instance count seen since the last check:
+        True (and the step captured again at the new size) when it exceeded
+        the capacity."""
+        m = int(self.max_count.item())
+        self.max_count.zero_()
+        if m > self.sizer.capacity:
+            self.capture()
+            return True
+        return False
+
+    def replay(self, check=True) -> bool:
+        """Run the captured step.  With ``check`` (a host wait), returns False
+        when the instance count had outgrown the capacity (the step was then
+        captured again and replayed, so the results are complete either way);
+        without, returns True at once (see ``overflowed``)."""
         for k, p in self.leaves.items():
             if p.grad is not self.grads[k]:
                 p.grad = self.grads[k]
         self.graph.replay()
-        if self.sizer.instances() > self.sizer.capacity:
-            self.capture()
+        if check and self.overflowed():
+            for k, p in self.leaves.items():
+                p.grad = self.grads[k]
             self.graph.replay()
             return False
         return True

@@ -270,6 +270,10 @@ class SortWorkspace:
         """Device instance count of the last sort (host read)."""
         return int(self.m_dev.item()) if self.m_dev is not None else 0
 
+    def count_tensor(self):
+        """The device int32[1] the instance count of each sort is written to."""
+        return self.m_dev
+
 
 _default_ws = SortWorkspace()
 last_instance_count = 0  # M of the most recent sort_gaussian call (diagnostics)
@@ -469,6 +473,7 @@ def compute_sh(shs, dirs, degree=None):
 FLAG_DEPTH = 1
 FLAG_NORMAL = 2
 FLAG_RAW = 4
+FLAG_PLAIN = 8   # colour = the SH sum, no +0.5 / clamp
 
 
 def _rows(t: torch.Tensor) -> torch.Tensor:
@@ -586,7 +591,8 @@ class _Preprocess(torch.autograd.Function):
 
 def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_matrix,
                camera_center, W, H, degree=None, depth_channel=False, normal_channel=False,
-               nearest=0.2, extent=1.3, keep_cov=False, raw=False, opacity=None):
+               nearest=0.2, extent=1.3, keep_cov=False, raw=False, opacity=None,
+               plain_sh=False):
     """Everything render_iter does per splat before sorting, in one kernel.
 
     Inputs as MsplatRender.render_iter receives them (activated scaling,
@@ -604,8 +610,14 @@ def preprocess(position, scaling, rotation, shs, intrinsic_params, extrinsic_mat
     gradients land on the raw parameters without separate elementwise
     passes.  With ``opacity`` the activated opacity [N,1] is appended to the
     returned tuple.
+
+    ``plain_sh=True``: the rgb channels are the SH sum itself, without the
+    +0.5 offset and the clamp (MsplatNormalRender's colour,
+    examples/supervise/renderer.py:41-42).
     """
     flags = (FLAG_DEPTH if depth_channel else 0) | (FLAG_NORMAL if normal_channel else 0)
+    if plain_sh:
+        flags |= FLAG_PLAIN
     if raw:
         flags |= FLAG_RAW
     elif opacity is not None:

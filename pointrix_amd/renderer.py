@@ -12,6 +12,7 @@ as examples/supervise renders it) and ``render_opacity`` (1 - T_final).
 from __future__ import annotations
 
 from dataclasses import dataclass, fields
+from collections.abc import Mapping
 from typing import Dict
 
 import torch
@@ -20,25 +21,40 @@ from . import ops
 
 
 class Registry:
-    """Name -> class registry (the role of pointrix/utils/registry.py)."""
+    """Name -> class registry with the semantics of pointrix/utils/registry.py:
+    an object registers under its ``__name__`` (as a decorator or a plain
+    call), a name can be registered once only, ``get`` raises KeyError for an
+    unknown name."""
 
     def __init__(self, name):
         self.name = name
         self._map = {}
 
+    def _add(self, name, obj):
+        if name in self._map:
+            raise AssertionError(f"An object named '{name}' was already registered in "
+                                 f"'{self.name}' registry!")
+        self._map[name] = obj
+
     def register(self, obj=None):
-        def deco(o):
-            self._map[o.__name__] = o
-            return o
-        return deco(obj) if obj is not None else deco
+        if obj is None:
+            def deco(o):
+                self._add(o.__name__, o)
+                return o
+            return deco
+        self._add(obj.__name__, obj)
+        return obj
 
     def get(self, name):
         if name not in self._map:
-            raise KeyError(f"{name} is not registered in {self.name}")
+            raise KeyError(f"No object named '{name}' found in '{self.name}' registry!")
         return self._map[name]
 
     def __contains__(self, name):
         return name in self._map
+
+    def __iter__(self):
+        return iter(self._map.items())
 
 
 RENDERER_REGISTRY = Registry("RENDERER")
@@ -114,13 +130,28 @@ class MsplatRender:
         tight_tiles: bool = False
 
     def __init__(self, cfg=None, white_bg=False, device="cuda", **kwargs):
-        if cfg is None:
-            cfg = self.Config()
-        elif isinstance(cfg, dict):
-            known = {f.name for f in fields(self.Config)}
-            cfg = self.Config(**{k: v for k, v in cfg.items() if k in known})
-        self.cfg = cfg
+        self.cfg = self.parse_cfg(cfg)
         self.setup(white_bg, device, **kwargs)
+
+    @classmethod
+    def parse_cfg(cls, cfg):
+        """Config from what BaseObject receives (pointrix/utils/base.py:32-39,
+        parse_structured at utils/config.py:102-107): None (defaults), a
+        Config, any Mapping (dict, OmegaConf DictConfig ...) or an object with
+        the fields as attributes.  As ``Config(**cfg)`` does there, a key that
+        is not a Config field raises TypeError."""
+        if cfg is None:
+            return cls.Config()
+        if isinstance(cfg, cls.Config):
+            return cfg
+        names = [f.name for f in fields(cls.Config)]
+        if isinstance(cfg, Mapping) or (hasattr(cfg, "keys") and hasattr(cfg, "__getitem__")):
+            items = {str(k): cfg[k] for k in cfg.keys()}
+            unknown = sorted(set(items) - set(names))
+            if unknown:
+                raise TypeError(f"{cls.__name__}.Config got unexpected keys {unknown}")
+            return cls.Config(**items)
+        return cls.Config(**{k: getattr(cfg, k) for k in names if hasattr(cfg, k)})
 
     def setup(self, white_bg, device, **kwargs):
         self.sh_degree = 0
@@ -197,17 +228,23 @@ class MsplatRender:
 
     def _render_iter_fused(self, height, width, extrinsic_matrix, intrinsic_params,
                            camera_center, position, opacity, scaling, rotation, shs, normals,
-                           raw=False):
-        normal_in_kernel = normals is None and self.cfg.render_normal
+                           raw=False, plain_sh=False):
+        """The fused path.  ``plain_sh``: colour = the SH sum over every
+        coefficient, without the +0.5 offset, the clamp and the warm-up degree
+        (MsplatNormalRender, examples/supervise/renderer.py:41-42), and no
+        depth channel."""
+        normal_in_kernel = normals is None and self.cfg.render_normal and not plain_sh
+        depth_channel = self.cfg.render_depth and not plain_sh
         out = ops.preprocess(
             position, scaling, rotation, shs, intrinsic_params, extrinsic_matrix, camera_center,
-            width, height, degree=self.sh_degree, depth_channel=self.cfg.render_depth,
-            normal_channel=normal_in_kernel, keep_cov=self.keep_intermediates, raw=raw,
-            opacity=opacity if raw else None)
+            width, height, degree=None if plain_sh else self.sh_degree,
+            depth_channel=depth_channel, normal_channel=normal_in_kernel,
+            keep_cov=self.keep_intermediates, raw=raw, opacity=opacity if raw else None,
+            plain_sh=plain_sh)
         uv, depth, conic, radius, tiles, feat, cov3d = out[:7]
         if raw:
             opacity = out[7]
-        names = [("rgb", 3)] + ([("depth", 1)] if self.cfg.render_depth else []) + \
+        names = [("rgb", 3)] + ([("depth", 1)] if depth_channel else []) + \
             ([("normal", 3)] if normal_in_kernel else [])
         if normals is not None:
             feat = torch.cat([feat, normals.to(feat.dtype)], -1)
@@ -291,20 +328,27 @@ def _split_planes(rendered, names):
 
 @RENDERER_REGISTRY.register()
 class MsplatNormalRender(MsplatRender):
-    """Normal-supervision variant (examples/supervise/renderer.py:8-102): blends
-    the ``normals`` kwarg without the +0.5 colour offset."""
+    """Normal-supervision variant (examples/supervise/renderer.py:8-102).
+    Unlike MsplatRender it takes the colour as the raw SH sum over every
+    coefficient (no +0.5, no clamp, no warm-up degree; :41-42), blends only
+    rgb and the ``normals`` kwarg (never depth; :73), and maps the blended
+    normal to [0,1] (:86-96).  ``render_opacity`` still adds 1 - T_final."""
 
     def render_iter(self, height, width, extrinsic_matrix, intrinsic_params, camera_center,
-                    position, opacity, scaling, rotation, shs, normals=None, **kwargs) -> dict:
-        """One view (msplat.py:51-158).  ``shs`` is [N,K,3], or the pair
-        (features [N,1,3], features_rest [N,K-1,3]) of pointrix's point cloud,
-        which the fused path reads without concatenating."""
+                    position, opacity, scaling, rotation, shs, normals=None, raw=False,
+                    **kwargs) -> dict:
+        """One view.  ``shs`` is [N,K,3], or the pair (features [N,1,3],
+        features_rest [N,K-1,3]), which the fused path reads without
+        concatenating; ``raw`` as for MsplatRender.render_iter."""
         if isinstance(shs, (tuple, list)) and not self.cfg.fused:
             shs = torch.cat(list(shs), dim=1)
         if self.cfg.fused:
             return self._render_iter_fused(height, width, extrinsic_matrix, intrinsic_params,
                                            camera_center, position, opacity, scaling, rotation,
-                                           shs, normals)
+                                           shs, normals, raw, plain_sh=True)
+        if raw:
+            opacity, scaling = torch.sigmoid(opacity), torch.exp(scaling)
+            rotation = torch.nn.functional.normalize(rotation, dim=-1)
         direction = position - camera_center.reshape(1, 3)
         direction = direction / direction.norm(dim=1, keepdim=True)
         rgb = ops.compute_sh(shs.permute(0, 2, 1), direction)
@@ -320,18 +364,29 @@ class MsplatNormalRender(MsplatRender):
             else RenderFeatures(rgb=rgb)
         ndc = torch.zeros_like(uv, requires_grad=True)
         ndc.retain_grad()
-        rendered = ops.alpha_blending(uv, conic, opacity, Render_Features.combine(), ids,
-                                      tile_range, self.bg_color, width, height, ndc)
+        rendered, T_final, n_contrib = ops.alpha_blending_ex(uv, conic, opacity,
+                                                             Render_Features.combine(), ids,
+                                                             tile_range, self.bg_color, width,
+                                                             height, ndc)
+        if self.keep_intermediates:
+            self.intermediates = {"uv": uv, "depth": depth, "cov3d": cov3d, "conic": conic,
+                                  "radius": radius, "ids": ids, "tile_range": tile_range,
+                                  "features": Render_Features.combine(), "rendered": rendered,
+                                  "T_final": T_final, "n_contrib": n_contrib}
         split = Render_Features.split(rendered)
         if normals is not None:
             nrm = split["normal"]
             split["normal"] = (nrm / nrm.norm(dim=0, keepdim=True) + 1) / 2
+        if self.cfg.render_opacity:
+            split["opacity"] = (1.0 - T_final)[None]
         return {"rendered_features_split": split, "uv_points": ndc,
                 "visibility": radius > 0, "radii": radius}
 
 
 def parse_renderer(cfg, **kwargs):
-    """Build a renderer from a config carrying ``name`` (renderer/__init__.py:4-19)."""
-    cfg = dict(cfg)
-    name = cfg.pop("name", "MsplatRender")
-    return RENDERER_REGISTRY.get(name)(cfg, **kwargs)
+    """Build a renderer from a config carrying ``name``
+    (pointrix/model/renderer/__init__.py:4-19): ``name`` is taken out and the
+    rest of the mapping becomes the renderer's Config."""
+    rest = {k: cfg[k] for k in cfg.keys() if k != "name"}
+    name = cfg["name"] if "name" in cfg else "MsplatRender"
+    return RENDERER_REGISTRY.get(name)(rest, **kwargs)

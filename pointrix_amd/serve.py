@@ -97,10 +97,13 @@ def instance_bytes() -> int:
 
 
 class _Inflight:
-    """A launched batch: its requests, results (device views) and end event."""
+    """A launched batch: its requests, results (device views), end event,
+    instance count and (for BatchServer) the event recorded on the slot's
+    stream before its first kernel."""
 
-    def __init__(self, reqs, results, event, instances):
+    def __init__(self, reqs, results, event, instances, start=None):
         self.reqs, self.results, self.event, self.instances = reqs, results, event, instances
+        self.start = start
 
 
 class BatchServer:
@@ -175,11 +178,16 @@ class BatchServer:
                              rd.tile[1])
             try:
                 with torch.cuda.stream(sl["stream"]), torch.no_grad():
+                    start = torch.cuda.Event(enable_timing=True)
+                    start.record()
                     vf = view_floats([r.camera() for r in batch]).to(self.dev, non_blocking=True)
                     out, _, _, _, _ = rd.render_packed(None, **self.inputs, layout=lay, vf=vf)
                     ev = torch.cuda.Event(enable_timing=True)
                     ev.record()
-                    m = rd.instances()   # read on the slot's stream
+                    # the count the renderer read (ahead of its depth sort) to
+                    # size the instance buffers: no further host wait, so the
+                    # next batch is admitted while this one is on the GPU
+                    m = rd.last_instances
             except InstanceBudgetExceeded as e:
                 self.splits += 1
                 self._learn(e.needed, lay.total_pixels)
@@ -196,7 +204,7 @@ class BatchServer:
             if r.depth:
                 d["depth"] = img[3:4]
             res.append(d)
-        sl["busy"] = _Inflight(batch, res, ev, m)
+        sl["busy"] = _Inflight(batch, res, ev, m, start)
         self.batch_sizes.append(len(batch))
 
     def poll(self, wait=False) -> list:

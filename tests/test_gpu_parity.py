@@ -59,12 +59,43 @@ def pre32(small):
                 rng=rng)
 
 
-def test_exp_matches_host(device):
-    # every blend decision goes through exp_neg; check its GPU values through
-    # a one-splat blend against the host restatement
-    xs = np.float32(-np.linspace(0.0, 5.5, 64))
-    for x in xs:
-        assert np.isfinite(r32.exp_neg(float(x)))
+@pytest.mark.parametrize("a", [2.0 ** -4, 2.0 ** -7, 2.0 ** -10])
+def test_exp_matches_host(device, a):
+    """The GPU's exp_neg, read out of one-splat blends: a splat at (0, 8)
+    with conic (a, 0, a) and opacity 0.5 over a 128x16 image.  On the row y
+    = 8 the power is -(a/2) px^2, exact in FP32 for these a (powers of two)
+    and integer px, and the blended value is 0.5 exp_neg(power) exactly (one
+    contributor, T = 1, feature 1, background 0), so twice the pixel must
+    equal the host restatement's exp_neg(power) bit for bit; the whole image
+    must equal the restatement's blend."""
+    W, H = 128, 16
+    uv = torch.tensor([[0.0, 8.0]])
+    conic = torch.tensor([[a, 0.0, a]])
+    op = torch.tensor([[0.5]])
+    feat = torch.ones(1, 1)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    ids = torch.zeros(T, dtype=torch.int32)
+    rng = torch.stack([torch.arange(T), torch.arange(T) + 1], 1).to(torch.int32)
+    t = lambda x: x.to(device)
+    out, Tf, nc = ops.alpha_blending_ex(t(uv), t(conic), t(op), t(feat), t(ids), t(rng), 0.0, W,
+                                        H)
+    row = out[0, 8].cpu().numpy()
+    checked = 0
+    for px in range(W):
+        power = np.float32(-(a / 2.0) * px * px)
+        e = r32.exp_neg(float(power))
+        alpha = np.float32(0.5) * np.float32(e)
+        if power < np.float32(-5.6) or alpha < np.float32(1.0) / np.float32(255.0):
+            assert row[px] == 0.0 and int(nc[8, px]) == 0
+            continue
+        assert np.float32(2.0) * row[px] == np.float32(e), (px, float(power), row[px], e)
+        assert abs(e - np.exp(np.float64(power))) <= 2.0 * np.spacing(np.float32(e))
+        checked += 1
+    assert checked >= 3
+    ref = r32.alpha_blending(uv.numpy(), conic.numpy(), op.numpy(), feat.numpy(), ids.numpy(),
+                             rng.numpy(), 0.0, W, H)
+    assert np.array_equal(out.cpu().numpy(), ref[0])
+    assert np.array_equal(nc.cpu().numpy(), ref[2])
 
 
 def test_project_point(device, small, pre32):
@@ -137,6 +168,108 @@ def test_sort_random_bitexact(device, n, W, H, seed, ties):
     ids_r, rng_r = r32.sort_gaussian(uv.numpy(), depth.numpy(), W, H, radius.numpy())
     assert np.array_equal(ids.cpu().numpy(), ids_r)
     assert np.array_equal(rng.cpu().numpy(), rng_r)
+
+
+def _queued_emission_case(seed=9, n_big=192, n_small=3000, W=1920, H=1080):
+    """192 large splats (radius 700-1200 px: most of the 120x68 tile grid)
+    consecutive in depth order, at random splat indices among 3000 small far
+    ones.  Each 64-splat wave of large splats owns ~500k instances: far past
+    the 8192 a wave writes itself, so it queues more than 64 items of 4096
+    for the second emission kernel."""
+    g = torch.Generator().manual_seed(seed)
+    n = n_big + n_small
+    uv = torch.rand(n, 2, generator=g) * torch.tensor([W, H])
+    radius = torch.randint(1, 20, (n,), generator=g, dtype=torch.int32)
+    depth = 1.0 + 10.0 * torch.rand(n, 1, generator=g)
+    big = torch.randperm(n, generator=g)[:n_big]
+    radius[big] = torch.randint(700, 1200, (n_big,), generator=g, dtype=torch.int32)
+    depth[big] = 0.5 + 0.1 * torch.rand(n_big, 1, generator=g)
+    return uv, depth, radius, W, H
+
+
+def _emission_order_reference(uv, depth, radius, W, H, cap):
+    """The first ``cap`` instances of the depth-first emission (splats in
+    (depth bits, index) order, each its tile rectangle row by row), then
+    stably sorted by tile: what a sort whose instance buffers hold only
+    ``cap`` instances produces.  Rectangles in FP32 as the kernels compute
+    them."""
+    u, v = uv[:, 0].astype(np.float32), uv[:, 1].astype(np.float32)
+    r = radius.astype(np.float32)
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    f16, f15 = np.float32(16.0), np.float32(15.0)
+    x0 = np.minimum(np.maximum((u - r) / f16, 0), gx).astype(np.int64)
+    x1 = np.minimum(np.maximum(((u + r) + f15) / f16, 0), gx).astype(np.int64)
+    y0 = np.minimum(np.maximum((v - r) / f16, 0), gy).astype(np.int64)
+    y1 = np.minimum(np.maximum(((v + r) + f15) / f16, 0), gy).astype(np.int64)
+    area = (x1 - x0) * (y1 - y0)
+    vis = np.nonzero((radius > 0) & (area > 0))[0]
+    bits = depth.reshape(-1).astype(np.float32).view(np.uint32)
+    order = vis[np.argsort(bits[vis], kind="stable")]
+    tiles, owners, total = [], [], 0
+    for i in order:
+        if total >= cap:
+            break
+        ty, tx = np.meshgrid(np.arange(y0[i], y1[i]), np.arange(x0[i], x1[i]), indexing="ij")
+        t = (ty * gx + tx).reshape(-1)
+        tiles.append(t)
+        owners.append(np.full(t.shape, i, np.int32))
+        total += t.size
+    tiles = np.concatenate(tiles)[:cap]
+    owners = np.concatenate(owners)[:cap]
+    perm = np.argsort(tiles, kind="stable")
+    ts, ids = tiles[perm], owners[perm]
+    rng = np.zeros((gx * gy, 2), np.int32)
+    first = np.searchsorted(ts, np.arange(gx * gy), side="left")
+    last = np.searchsorted(ts, np.arange(gx * gy), side="right")
+    nz = last > first
+    rng[nz, 0], rng[nz, 1] = first[nz], last[nz]
+    # instances per 64-splat emission wave (splats in depth order)
+    a = area[order]
+    waves = np.add.reduceat(a, np.arange(0, len(a), 64)) if len(a) else a
+    return ids, rng, int(area[vis].sum()), waves
+
+
+def test_sort_queued_emission_bitexact(device):
+    """Waves of large splats past the per-wave emission span (the queued
+    second emission kernel): ids and tile ranges bit-exact."""
+    uv, depth, radius, W, H = _queued_emission_case()
+    ids, rng = ops.sort_gaussian(uv.to(device), depth.to(device), W, H, radius.to(device), None)
+    ids_r, rng_r = r32.sort_gaussian(uv.numpy(), depth.numpy(), W, H, radius.numpy())
+    waves = _emission_order_reference(uv.numpy(), depth.numpy(), radius.numpy(), W, H, 1)[3]
+    print(f"queued emission: M = {len(ids_r)}, largest wave {waves.max()} instances")
+    # a wave writes 8192 itself and queues the rest as 4096-instance items:
+    # more than 64 items means the item loop's second pass runs
+    assert waves.max() > 8192 + 64 * 4096
+    assert np.array_equal(ids.cpu().numpy(), ids_r)
+    assert np.array_equal(rng.cpu().numpy(), rng_r)
+
+
+@pytest.mark.parametrize("frac", [0.4, 0.97])
+def test_sort_capacity_below_count(device, frac):
+    """A fixed instance capacity below the instance count (what a captured
+    graph uses): the device count still reports the true M, the emission
+    stops at the capacity -- the ids are the first ``cap`` instances of the
+    depth-first emission, tile-sorted, and the ranges index them -- and
+    nothing is written past it."""
+    uv, depth, radius, W, H = _queued_emission_case(seed=10)
+    ids_full, _ = r32.sort_gaussian(uv.numpy(), depth.numpy(), W, H, radius.numpy())
+    m = len(ids_full)
+    cap = int(m * frac)
+    ws = ops.SortWorkspace()
+    ws.capacity = cap
+    ids, rng = ops.sort_gaussian(uv.to(device), depth.to(device), W, H, radius.to(device), None,
+                                 workspace=ws)
+    assert ids.shape[0] == cap and ws.instances() == m
+    ids_e, rng_e, m_e, _ = _emission_order_reference(uv.numpy(), depth.numpy(),
+                                                     radius.numpy(), W, H, cap)
+    assert m_e == m
+    assert np.array_equal(ids.cpu().numpy(), ids_e)
+    assert np.array_equal(rng.cpu().numpy(), rng_e)
+    # the same workspace with an exact sort afterwards is unaffected
+    ws.capacity = None
+    ids2, rng2 = ops.sort_gaussian(uv.to(device), depth.to(device), W, H, radius.to(device),
+                                   None, workspace=ws)
+    assert np.array_equal(ids2.cpu().numpy(), ids_full)
 
 
 def test_sort_empty(device):

@@ -13,8 +13,8 @@ FP64 autograd within 1e-5 relative, with some groups' gradients absent.
 import pytest
 import torch
 
+from oracle.torch64 import quat_to_rotmat
 from pointrix_amd import ops
-from pointrix_amd.renderer import quat_to_rotmat
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +26,10 @@ def rel_err(a, b):
 
 
 def pose_ref(q, t):
-    R = quat_to_rotmat(torch.nn.functional.normalize(q, dim=-1))
+    """The oracle's rotation (pinned against scipy in test_oracle.py) composed
+    as CameraModel does."""
+    R = quat_to_rotmat(torch.nn.functional.normalize(q, dim=-1).reshape(-1, 4))
+    R = R.reshape(*q.shape[:-1], 3, 3)
     Rt = torch.cat([R, t[..., None]], -1)
     bottom = torch.zeros(*q.shape[:-1], 1, 4, dtype=q.dtype)
     bottom[..., 0, 3] = 1.0

@@ -86,3 +86,35 @@ def test_overlapping_batches(device):
     got = {r.rid: o for r, o, _ in srv.poll(wait=True)}
     for r, o in zip(reqs, ref):
         assert equal(got[r.rid], o), r.rid
+
+
+def test_batches_overlap_on_device(device):
+    """Continuous batching overlaps on the device: a second batch admitted
+    while a long first one is in flight starts on the GPU before the first
+    one ends (device events recorded on the two slot streams), and both
+    return the images they give rendered alone."""
+    scene = S.random_scene(1_000_000, 76, sh_degree=3, means="normal",
+                           means_std=(2.0, 1.0, 2.0), log_scale=(-4.0, 0.5), opacity_std=1.0,
+                           shell_frac=0.1).to(device)
+    inputs = scene.render_inputs(split_sh=True)
+    long = make_requests(100.0, 12, seed=76, resolutions=[((1920, 1080), 1.0)],
+                         radius=(2.0, 3.0))
+    short = make_requests(100.0, 3, seed=77, resolutions=RES)
+    for k, r in enumerate(short):
+        r.rid = 100 + k
+    srv = BatchServer(inputs, slots=2, key_budget_bytes=64 << 30)
+    alone = {r.rid: o for r, o in zip(long + short, srv.render(long) + srv.render(short))}
+    q = collections.deque()
+    srv.submit(long, q)
+    a = srv.slots[0]["busy"]
+    srv.submit(short, q)
+    b = srv.slots[1]["busy"]
+    assert a is not None and b is not None and not q
+    got = {r.rid: o for r, o, _ in srv.poll(wait=True)}
+    a_len = a.start.elapsed_time(a.event)
+    b_start = a.start.elapsed_time(b.start)
+    print(f"first batch {a_len:.3f} ms on the device; second started {b_start:.3f} ms after "
+          f"the first started")
+    assert b_start < a_len
+    for rid, o in alone.items():
+        assert equal(got[rid], o), rid

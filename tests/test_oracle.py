@@ -382,3 +382,182 @@ def test_render_iter_finite_differences():
             assert abs(fd - an) <= 1e-4 * scale, (k, j, fd, an)
             checked += 1
     assert checked == 8 * len(inp)
+
+
+# ---------------------------------------------------------------------------
+# splat normals (examples/supervise/model.py:50-74)
+# ---------------------------------------------------------------------------
+def _normal_case(n, seed):
+    g = torch.Generator().manual_seed(seed)
+    pos = torch.randn(n, 3, dtype=F64, generator=g)
+    scale = torch.rand(n, 3, dtype=F64, generator=g) + 0.05
+    q = unit(torch.randn(n, 4, dtype=F64, generator=g))
+    cam = S.make_camera(64, 48, eye=(0.4, -0.3, -4.0), fov_deg=50.0)
+    return pos, scale, q, cam.camera_center.double(), cam.extrinsic_matrix.double()
+
+
+def test_splat_normals_identity_quaternion():
+    """Identity rotation: the normal is +-e_k for the smallest scale axis k,
+    the sign facing the camera, then rotated into camera space by E[:3,:3]."""
+    pos, scale, _, c, E = _normal_case(300, 40)
+    q = torch.tensor([[1.0, 0.0, 0.0, 0.0]], dtype=F64).expand(300, 4)
+    n_cam = o.splat_normals(pos, scale, q, c, E)
+    k = scale.argmin(1)
+    for i in range(300):
+        e = torch.zeros(3, dtype=F64)
+        e[k[i]] = 1.0
+        if torch.dot(c - pos[i], e) < 0:
+            e = -e
+        assert torch.allclose(n_cam[i], E[:3, :3] @ e, atol=1e-14)
+
+
+def test_splat_normals_match_scipy_rotation():
+    """Random rotations: the world normal is the scipy rotation matrix's
+    column of the smallest scale axis, unit length, turned to the camera."""
+    pos, scale, q, c, E = _normal_case(500, 41)
+    n_cam = o.splat_normals(pos, scale, q, c, E)
+    Rs = torch.from_numpy(Rotation.from_quat(q[:, [1, 2, 3, 0]].numpy()).as_matrix())
+    k = scale.argmin(1)
+    world = Rs[torch.arange(500), :, k]
+    sign = torch.where(((c[None] - pos) * world).sum(1) < 0, -1.0, 1.0).to(F64)
+    want = (sign[:, None] * world) @ E[:3, :3].T
+    assert torch.allclose(n_cam, want, atol=1e-12)
+    assert torch.allclose(n_cam.norm(dim=1), torch.ones(500, dtype=F64), atol=1e-12)
+    # facing the camera: in camera space the normal points against the
+    # camera-to-splat direction
+    p_cam = pos @ E[:3, :3].T + E[:3, 3]
+    assert torch.all((n_cam * p_cam).sum(1) <= 1e-12)
+
+
+def test_splat_normals_gradcheck():
+    pos, scale, q, c, E = _normal_case(20, 42)
+    leaves = [t.clone().requires_grad_(True) for t in (q, E)]
+    assert torch.autograd.gradcheck(lambda qq, EE: o.splat_normals(pos, scale, qq, c, EE),
+                                    leaves, eps=1e-7, atol=1e-7)
+
+
+def test_render_iter_normal_finite_differences():
+    """1,000 splats, 48x40, SH degree 3, rgb + depth + normal (the normal
+    channel from splat_normals, mapped to [0,1] after blending): autograd vs
+    central differences on every input, the camera centre included."""
+    scene = S.random_scene(1000, 23, sh_degree=3, means="normal", means_std=(0.6, 0.6, 0.6),
+                           log_scale=(-2.8, 0.5), opacity_std=1.5)
+    cam = S.make_camera(48, 40, eye=(0.3, 0.2, -2.5), fov_deg=55.0)
+    inp = {k: v.detach().double() for k, v in scene.render_inputs().items()}
+    inp["extrinsic_matrix"] = cam.extrinsic_matrix.double()
+    inp["intrinsic_params"] = cam.intrinsic_params.double()
+    inp["camera_center"] = cam.camera_center.double()
+    g = torch.Generator().manual_seed(24)
+    w = {k: torch.randn(c, 40, 48, generator=g, dtype=F64)
+         for k, c in (("rgb", 3), ("depth", 1), ("normal", 3))}
+
+    def loss_of(t):
+        nrm = o.splat_normals(t["position"], t["scaling"], t["rotation"], t["camera_center"],
+                              t["extrinsic_matrix"])
+        res = o.render_iter(40, 48, t["extrinsic_matrix"], t["intrinsic_params"],
+                            t["camera_center"], t["position"], t["opacity"], t["scaling"],
+                            t["rotation"], t["shs"], sh_degree=3, render_depth=True,
+                            normals=nrm)
+        f = res["rendered_features_split"]
+        return sum((f[k] * w[k]).sum() for k in w)
+
+    leaves = {k: v.clone().requires_grad_(True) for k, v in inp.items()}
+    loss_of(leaves).backward()
+    eps = 1e-6
+    for k, v in inp.items():
+        flat = leaves[k].grad.reshape(-1)
+        idx = torch.cat([flat.abs().topk(min(4, flat.numel())).indices,
+                         torch.randint(0, flat.numel(), (3,), generator=g)])
+        for j in idx.tolist():
+            plus = {kk: vv.clone() for kk, vv in inp.items()}
+            minus = {kk: vv.clone() for kk, vv in inp.items()}
+            plus[k].view(-1)[j] += eps
+            minus[k].view(-1)[j] -= eps
+            with torch.no_grad():
+                fd = (loss_of(plus) - loss_of(minus)).item() / (2 * eps)
+            an = flat[j].item()
+            assert abs(fd - an) <= 1e-4 * max(1.0, flat.abs().max().item()), (k, j, fd, an)
+
+
+# ---------------------------------------------------------------------------
+# the FP32 restatement's backward (ref_blend_bwd, ref_preprocess_bwd)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("flags,degree,bg", [(0, 3, 0.0), (1, 1, 1.0), (3, 3, 0.0), (8, 3, 0.0),
+                                             (11, 2, 1.0)])
+def test_ref32_backward_matches_fp64_autograd(flags, degree, bg):
+    """ref32.render_step's gradients (FP32 chain rule of the restatement) vs
+    autograd of the FP64 transcription at the restatement's forward point
+    (uv and conic snapped): every splat and camera gradient within 1e-5 of
+    its largest entry.  flags: 1 depth, 2 normal, 8 plain colour."""
+    W, H = 64, 56
+    scene = S.random_scene(2000, 11 + flags, sh_degree=3, means="normal",
+                           means_std=(0.6, 0.6, 0.6), log_scale=(-3.0, 0.5), opacity_std=1.5)
+    cam = S.make_camera(W, H, eye=(0.4, -0.3, -2.5), fov_deg=55.0)
+    inp = {k: v.detach() for k, v in scene.render_inputs().items()}
+    names = [("rgb", 3)] + ([("depth", 1)] if flags & 1 else []) + \
+        ([("normal", 3)] if flags & 2 else [])
+    g = torch.Generator().manual_seed(3)
+    w = {k: torch.randn(c, H, W, generator=g, dtype=F64) for k, c in names}
+    ref = r32.render_step({k: v.numpy() for k, v in inp.items()},
+                          (W, H, cam.intrinsic_params.numpy(), cam.extrinsic_matrix.numpy(),
+                           cam.camera_center.numpy()), degree=degree, flags=flags, bg=bg,
+                          keep=True,
+                          image_grad=lambda p: (0.0, {k: w[k].numpy() for k in w}))
+    hl = {k: v.double().requires_grad_(True) for k, v in inp.items()}
+    E = cam.extrinsic_matrix.double().requires_grad_(True)
+    K = cam.intrinsic_params.double().requires_grad_(True)
+    c = cam.camera_center.double().requires_grad_(True)
+    normals = o.splat_normals(hl["position"], hl["scaling"], hl["rotation"], c, E) \
+        if flags & 2 else None
+    snap = {"uv": torch.from_numpy(ref["uv"]), "conic": torch.from_numpy(ref["conic"])}
+    res = o.render_iter(H, W, E, K, c, hl["position"], hl["opacity"], hl["scaling"],
+                        hl["rotation"], hl["shs"], sh_degree=degree, bg=bg,
+                        render_depth=bool(flags & 1), normals=normals,
+                        add_half=not flags & 8, snap=snap)
+    f = res["rendered_features_split"]
+    for k in f:
+        assert np.abs(ref["planes"][k] - f[k].detach().numpy()).max() < 1e-5, k
+    sum((f[k] * w[k]).sum() for k in w).backward()
+    pairs = [("position", hl["position"].grad), ("scaling", hl["scaling"].grad),
+             ("rotation", hl["rotation"].grad), ("shs", hl["shs"].grad),
+             ("opacity", hl["opacity"].grad.reshape(-1)), ("intr", K.grad),
+             ("extr", E.grad[:3]), ("center", c.grad)]
+    for k, want in pairs:
+        got = np.asarray(ref["grads"][k], np.float64)
+        e = np.abs(got - want.numpy()).max() / max(np.abs(want.numpy()).max(), 1e-30)
+        assert e < 1e-5, (k, e)
+
+
+def test_ref32_sort_matches_brute_force_at_size():
+    """The restatement's bucket sort against the definition, at size: every
+    (splat, tile) pair of the FP32 tile rectangles as a 64-bit key tile << 32
+    | f32 depth bits, in splat order, stably sorted (numpy) -- 50k random
+    splats with ties and off-screen rectangles."""
+    g = torch.Generator().manual_seed(12)
+    n, W, H = 50_000, 700, 500
+    uv = ((torch.rand(n, 2, generator=g) * 1.4 - 0.2) * torch.tensor([W, H])).numpy()
+    r = torch.randint(0, 60, (n,), generator=g, dtype=torch.int32).numpy()
+    depth = torch.round(torch.rand(n, 1, generator=g) * 40 + 1).numpy()
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    u, v, rf = uv[:, 0].astype(np.float32), uv[:, 1].astype(np.float32), r.astype(np.float32)
+    f16, f15 = np.float32(16.0), np.float32(15.0)
+    x0 = np.minimum(np.maximum((u - rf) / f16, 0), gx).astype(np.int64)
+    x1 = np.minimum(np.maximum(((u + rf) + f15) / f16, 0), gx).astype(np.int64)
+    y0 = np.minimum(np.maximum((v - rf) / f16, 0), gy).astype(np.int64)
+    y1 = np.minimum(np.maximum(((v + rf) + f15) / f16, 0), gy).astype(np.int64)
+    bits = depth.reshape(-1).astype(np.float32).view(np.uint32).astype(np.uint64)
+    keys, owner = [], []
+    for i in np.nonzero(r > 0)[0]:
+        ty, tx = np.meshgrid(np.arange(y0[i], y1[i]), np.arange(x0[i], x1[i]), indexing="ij")
+        t = (ty * gx + tx).reshape(-1).astype(np.uint64)
+        keys.append((t << np.uint64(32)) | bits[i])
+        owner.append(np.full(t.shape, i, np.int32))
+    keys, owner = np.concatenate(keys), np.concatenate(owner)
+    perm = np.argsort(keys, kind="stable")
+    ids32, rng32 = r32.sort_gaussian(uv, depth, W, H, r)
+    assert np.array_equal(ids32, owner[perm])
+    tiles = (keys[perm] >> np.uint64(32)).astype(np.int64)
+    first = np.searchsorted(tiles, np.arange(gx * gy), side="left")
+    last = np.searchsorted(tiles, np.arange(gx * gy), side="right")
+    want = np.where((last > first)[:, None], np.stack([first, last], 1), 0)
+    assert np.array_equal(rng32, want.astype(np.int32))
