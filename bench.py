@@ -23,6 +23,7 @@ import os
 import sys
 import time
 
+import numpy as np
 import torch
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -31,13 +32,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-# workload, each scaled by the counter's reading for a copy of known size
-# (fetch x2.0, write x1.0); summary in profiles/alpha_blending_bwd_full3m_pmc.md
-PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 101689 * 1024 * 2 + 94390 * 1024,
-                     "alpha_blending_fwd": 77507 * 1024 * 2 + 51523 * 1024,
-                     "preprocess_bwd": 294568 * 1024 * 2 + 680441 * 1024,
-                     "preprocess_fwd": 255534 * 1024 * 2 + 175800 * 1024}
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+# full3m workload, each scaled by the counter's reading for a copy of known
+# size; summary and commands in PMC_SOURCE
+PMC_SOURCE = "profiles/blend_full3m_pmc.md"
+PMC_TRAFFIC_BYTES = {}
+# the blend kernels' instruction counters (rocprofv3 --pmc SQ_*) on the
+# full3m workload, from the same summary
+VALU_PMC = {}
 
 
 def rotmat_to_quat(R):
@@ -203,36 +205,275 @@ def algorithmic_bytes(N, K, C, P, T, Nv, M):
     }
 
 
-def cpu_baseline(workload_fn, div, seed=0):
-    """FP64 oracle fwd+bwd on a random 1/div subset of the splats (same camera,
-    rgb + depth + normal, gradients into splats and camera), scaled to steps/s
-    of the full scene by the subset factor."""
-    from oracle import torch64 as o64
-    scene, cams = workload_fn(seed)
-    cam = cams[0]
-    n = len(scene)
-    g = torch.Generator().manual_seed(seed + 7)
-    idx = torch.randperm(n, generator=g)[: max(1, n // div)]
-    inp = {k: v[idx].detach().double().requires_grad_(True)
-           for k, v in scene.render_inputs().items()}
-    E = cam.extrinsic_matrix.double().requires_grad_(True)
-    K = cam.intrinsic_params.double().requires_grad_(True)
-    c = cam.camera_center.double().requires_grad_(True)
-    threads = torch.get_num_threads()
-    t0 = time.perf_counter()
-    normals = o64.splat_normals(inp["position"], inp["scaling"], inp["rotation"], c, E)
-    ref = o64.render_iter(cam.height, cam.width, E, K, c, inp["position"], inp["opacity"],
-                          inp["scaling"], inp["rotation"], inp["shs"], sh_degree=3,
-                          render_depth=True, normals=normals)
-    loss = sum(v.abs().mean() for v in ref["rendered_features_split"].values())
+# VALU issue ceiling: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one FP32
+# operation per lane per cycle (packed FP32 is not generated)
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
+# FP32 VALU operations per (pixel, splat) pair of the blend recurrences as the
+# oracle writes them (oracle/ref32.c ref_blend / ref_blend_bwd): one per
+# arithmetic operator, compare, min / max or select; exp_neg expanded into its
+# 14 operations (clamp, scale, round, 2 + 7 fma, exponent convert and scale);
+# a division counted once.
+#   forward, blended pair: dx, dy 2; power 6; two cut tests 2; exp 14; alpha
+#     (mul, min) 2; alpha test 1; T test (sub, mul, cmp) 3; weight 1; C
+#     feature fma; state selects 2 -> 33 + C
+#   backward, replayed pair: the forward's alpha 27; T recovery (sub, div) 2;
+#     dfeature (C mul-add) and the colour-behind dot product (C fma) 2C; its
+#     recurrence 4; dL/dalpha 4; clamp test 1; dpower 1; du, dv, da, db, dc,
+#     dopacity record terms 24 -> 63 + 2C
+#   rejected pair (either walk): dx, dy, power, one test -> 9
+def blend_ops(C, blended, examined):
+    rej = max(examined - blended, 0)
+    return {"alpha_blending_fwd": (33 + C) * blended + 9 * rej,
+            "alpha_blending_bwd": (63 + 2 * C) * blended + 9 * rej}
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Device-to-device copy of ``nbytes`` (torch copy_), timed with events on
+    the current stream: bytes read + written per second (GB/s), median of
+    ``reps`` after 3 warm-up copies."""
+    dev = torch.device("cuda:0")
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    times.sort()
+    t = times[len(times) // 2]
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(2 * nbytes / t / 1e9, 1)
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.abs(b).max() if b.size else 0.0
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0)) if a.size else 0.0
+
+
+L1_WEIGHTS = {"rgb": 1.0, "depth": 0.1, "normal": 0.1}
+
+
+def gpu_reference_step(renderer, scene, cam, targets):
+    """One eager step of the benchmark's render + loss with the render_iter
+    inputs (activated splat tensors, the composed extrinsic, intrinsics and
+    centre) taken as leaves: -> numpy inputs, camera, planes, intermediates
+    and gradients, for the comparison with the host restatement."""
+    with torch.no_grad():
+        inp = scene.render_inputs(split_sh=True)
+        ck = cam.render_kwargs()
+    leaves = {k: (v.detach().clone().requires_grad_(True) if torch.is_tensor(v) else
+                  [t.detach().clone().requires_grad_(True) for t in v]) for k, v in inp.items()}
+    cl = {k: ck[k].detach().clone().requires_grad_(True)
+          for k in ("extrinsic_matrix", "intrinsic_params", "camera_center")}
+    renderer.keep_intermediates = True
+    res = renderer.render_iter(ck["height"], ck["width"], **cl, **leaves)
+    f = res["rendered_features_split"]
+    loss = l1_sum([(f[k], targets[k], L1_WEIGHTS[k]) for k in ("rgb", "depth", "normal")])
     loss.backward()
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / (dt * div), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 PyTorch-CPU oracle (oracle/torch64.py), fwd+bwd of rgb+depth+normal "
-                      f"with splat and camera gradients, on a random 1/{div} subset "
-                      f"({len(idx)} splats) of the scene with the same camera; {dt:.1f} s "
-                      f"measured, scaled by {div}; torch threads={threads}, "
-                      f"os.cpu_count()={os.cpu_count()}"}
+    im = {k: v.detach().cpu().numpy() for k, v in renderer.intermediates.items()}
+    renderer.keep_intermediates = False
+    renderer.intermediates = None
+    host = {k: v.detach().cpu().numpy() for k, v in leaves.items() if k != "shs"}
+    host["shs"] = torch.cat([t.detach().cpu() for t in leaves["shs"]], 1).numpy()
+    grads = {k: leaves[k].grad.cpu().numpy() for k in ("position", "scaling", "rotation",
+                                                        "opacity")}
+    grads["shs"] = torch.cat([t.grad.cpu() for t in leaves["shs"]], 1).numpy()
+    grads["extr"] = cl["extrinsic_matrix"].grad.cpu().numpy()[:3]
+    grads["intr"] = cl["intrinsic_params"].grad.cpu().numpy().reshape(-1)[:4]
+    grads["center"] = cl["camera_center"].grad.cpu().numpy().reshape(-1)
+    camera = (ck["width"], ck["height"], cl["intrinsic_params"].detach().cpu().numpy(),
+              cl["extrinsic_matrix"].detach().cpu().numpy(),
+              cl["camera_center"].detach().cpu().numpy())
+    planes = {k: v.detach().cpu().numpy() for k, v in f.items()}
+    return host, camera, planes, im, grads, float(loss.detach())
+
+
+def restatement_leg(renderer, scene, cam, targets, reps=5):
+    """The whole step (render, the same weighted L1 losses, backward into the
+    splats and the camera) by the FP32 C restatement (oracle/ref32.c, OpenMP on
+    this host's cores) on the GPU step's own inputs: timed in full (median of
+    ``reps`` after one warm-up) as the CPU baseline, compared with the GPU step
+    (forward bit for bit, gradients relative to the largest), and its exact
+    blend work counts (the restatement's walk is the kernels' walk)."""
+    from oracle import ref32 as r32
+    host, camera, planes, im, grads, _ = gpu_reference_step(renderer, scene, cam, targets)
+    tg = {k: v.detach().cpu().numpy() for k, v in targets.items()}
+
+    def image_grad(p):
+        out, loss = {}, 0.0
+        for k, x in p.items():
+            w = np.float32(L1_WEIGHTS[k] / x.size)
+            out[k] = (np.sign(x - tg[k]).astype(np.float32) * w).astype(np.float32)
+            loss += L1_WEIGHTS[k] * float(np.abs(x - tg[k]).mean())
+        return loss, out
+
+    def run(keep=False):
+        return r32.render_step(host, camera, degree=3, flags=3, keep=keep,
+                               image_grad=image_grad)
+
+    ref = run(keep=True)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = _median(times)
+    W, H = camera[0], camera[1]
+    exact = {k: bool(np.array_equal(im[k].reshape(np.shape(ref[k])), ref[k]))
+             for k in ("uv", "depth", "conic", "radius", "ids", "tile_range", "n_contrib",
+                       "T_final")}
+    exact.update({f"image_{k}": bool(np.array_equal(planes[k], ref["planes"][k]))
+                  for k in planes})
+    gerr = {k: float(f"{_rel(grads[k].reshape(np.shape(ref['grads'][k])), ref['grads'][k]):.3g}")
+            for k in ("position", "scaling", "rotation", "opacity", "shs", "extr", "intr",
+                      "center")}
+    examined, blended, prefix = r32.blend_pairs(ref["uv"], ref["conic"], host["opacity"],
+                                                ref["ids"], ref["tile_range"], W, H)
+    threads = r32.num_threads()
+    cpu = {"value": round(1.0 / t, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+           "mpixel_per_s": round(W * H / 1e6 / t, 3),
+           "sample": f"the full step at its full size: the FP32 C restatement of the path "
+                     f"(oracle/ref32.c: preprocess, binning, blending, their backward, camera "
+                     f"gradients) with the same weighted L1 losses, on the GPU step's own "
+                     f"inputs; median of {reps} steps after one warm-up "
+                     f"({', '.join(f'{x:.2f}' for x in times)} s); OpenMP threads={threads}, "
+                     f"os.cpu_count()={os.cpu_count()}"}
+    parity = {"reference": "FP32 C restatement at full size, same inputs",
+              "bit_exact": exact, "grad_rel_err": gerr}
+    pairs = {"examined": examined, "blended": blended, "n_contrib_sum": prefix}
+    return cpu, parity, pairs
+
+
+def fp64_cpu10k(reps=5):
+    """The FP64 PyTorch-CPU transcription (oracle/torch64.py) of the 10k-splat
+    256x256 SH-0 L1 step, timed in full: median of ``reps`` after one warm-up,
+    torch threads as set on this host."""
+    from oracle import torch64 as o64
+    from pointrix_amd import scene as S
+    scene, cams = S.cpu10k()
+    cam = cams[0]
+    target = torch.rand(3, cam.height, cam.width,
+                        generator=torch.Generator().manual_seed(5)).double()
+    times = []
+    for i in range(reps + 1):
+        inp = {k: v.detach().double().requires_grad_(True)
+               for k, v in scene.render_inputs().items()}
+        t0 = time.perf_counter()
+        ref = o64.render_iter(cam.height, cam.width, cam.extrinsic_matrix.double(),
+                              cam.intrinsic_params.double(), cam.camera_center.double(),
+                              inp["position"], inp["opacity"], inp["scaling"], inp["rotation"],
+                              inp["shs"], sh_degree=0)
+        (ref["rendered_features_split"]["rgb"] - target).abs().mean().backward()
+        if i:
+            times.append(time.perf_counter() - t0)
+    t = _median(times)
+    return {"workload": "cpu10k: 10k splats SH0, 256x256, L1", "value": round(1.0 / t, 3),
+            "unit": "steps/s", "mpixel_per_s": round(256 * 256 / 1e6 / t, 4),
+            "threads": torch.get_num_threads(), "cpu_count": os.cpu_count(),
+            "steps_s": [round(x, 3) for x in times]}
+
+
+def _fp64_errors(inp, cam, flags, sh_degree, bg, dev, snapped):
+    """GPU render_iter vs the FP64 transcription on the same leaves: max |image
+    error| per output plane, max relative error per gradient (backward of a
+    seeded random weighting of every plane)."""
+    from oracle import torch64 as o64
+    from pointrix_amd.renderer import MsplatRender
+    r = MsplatRender({"render_depth": bool(flags & 1), "render_normal": bool(flags & 2),
+                      "render_opacity": bool(flags & 4)}, white_bg=bool(bg), device="cuda")
+    r.sh_degree = sh_degree
+    r.keep_intermediates = True
+    camk = {"extrinsic_matrix": cam.extrinsic_matrix, "intrinsic_params": cam.intrinsic_params,
+            "camera_center": cam.camera_center}
+    dl = {k: v.to(dev).requires_grad_(True) for k, v in {**inp, **camk}.items()}
+    res = r.render_iter(cam.height, cam.width, **dl)
+    imgs = res["rendered_features_split"]
+    g = torch.Generator().manual_seed(1)
+    wts = {k: torch.randn(v.shape, generator=g) for k, v in imgs.items()}
+    sum((imgs[k] * wts[k].to(dev)).sum() for k in imgs).backward()
+    snap = None
+    if snapped:
+        snap = {"uv": r.intermediates["uv"].detach().cpu(),
+                "conic": r.intermediates["conic"].detach().cpu()}
+    hl = {k: v.double().requires_grad_(True) for k, v in {**inp, **camk}.items()}
+    normals = None
+    if flags & 2:
+        normals = o64.splat_normals(hl["position"], hl["scaling"], hl["rotation"],
+                                    hl["camera_center"], hl["extrinsic_matrix"])
+    ref = o64.render_iter(cam.height, cam.width, hl["extrinsic_matrix"], hl["intrinsic_params"],
+                          hl["camera_center"], hl["position"], hl["opacity"], hl["scaling"],
+                          hl["rotation"], hl["shs"], sh_degree=sh_degree, bg=bg,
+                          render_depth=bool(flags & 1), normals=normals,
+                          render_opacity=bool(flags & 4), snap=snap)
+    rimgs = ref["rendered_features_split"]
+    sum((rimgs[k] * wts[k].double()).sum() for k in rimgs).backward()
+    img = {k: float(f"{(imgs[k].detach().cpu().double() - rimgs[k].detach()).abs().max():.3g}")
+           for k in imgs}
+    names = {"extrinsic_matrix": "extr", "intrinsic_params": "intr", "camera_center": "center"}
+    def g_of(t):  # a leaf the output does not depend on (the centre at SH 0) has none
+        return np.zeros(tuple(t.shape)) if t.grad is None else t.grad.detach().cpu().double().numpy()
+
+    grad = {names.get(k, k): float(f"{_rel(g_of(dl[k]), g_of(hl[k])):.3g}") for k in dl}
+    return {"image_max_abs_err": img, "grad_rel_err": grad}
+
+
+def crop2000(scene_cpu, cam, n=2000, size=64, seed=0):
+    """2,000 splats of the scene whose footprint meets a size x size window at
+    the centre of the camera's image, and the camera cropped to that window
+    (principal point shifted): the FP64-checkable crop of the full step."""
+    from oracle import ref32 as r32
+    from pointrix_amd import scene as S
+    inp = {k: v.detach() for k, v in scene_cpu.render_inputs().items()}
+    K = cam.intrinsic_params.clone().reshape(-1)
+    E = cam.extrinsic_matrix
+    W, H = cam.width, cam.height
+    uv, d = r32.project_point(inp["position"].numpy(), K.numpy(), E.numpy(), W, H, 0.2)
+    vis = d != 0
+    cov = r32.compute_cov3d(inp["scaling"].numpy(), inp["rotation"].numpy(), vis)
+    _, rad, _ = r32.ewa_project(inp["position"].numpy(), cov, K.numpy(), E.numpy(), uv, W, H, vis)
+    x0, y0 = (W - size) // 2, (H - size) // 2
+    hit = ((rad > 0) & (uv[:, 0] + rad >= x0) & (uv[:, 0] - rad < x0 + size) &
+           (uv[:, 1] + rad >= y0) & (uv[:, 1] - rad < y0 + size))
+    idx = np.flatnonzero(hit)
+    g = np.random.default_rng(seed)
+    idx = np.sort(g.choice(idx, size=min(n, len(idx)), replace=False))
+    sub = {k: v[torch.from_numpy(idx)].contiguous() for k, v in inp.items()}
+    Kc = K.clone()
+    Kc[2] -= x0
+    Kc[3] -= y0
+    crop = S.Camera(size, size, E.clone(), Kc.reshape(cam.intrinsic_params.shape),
+                    cam.camera_center.clone())
+    return sub, crop
+
+
+def fp64_errors(scene_cpu, cam, dev):
+    """Error of every output against the FP64 transcription: the 2,000-splat
+    64x64 crop of the full step (rgb, depth, normal, opacity; SH 3; white
+    background; gradients into the splats and the camera), unsnapped (both
+    sides from the same FP32 leaves, which includes FP32 pixel-coordinate
+    rounding) and snapped (FP64 evaluated at the GPU's uv and conic), and the
+    10k-splat SH-0 256x256 workload unsnapped."""
+    from pointrix_amd import scene as S
+    sub, crop = crop2000(scene_cpu, cam)
+    out = {"crop2000_64x64": {"unsnapped": _fp64_errors(sub, crop, 7, 3, 1.0, dev, False),
+                              "snapped": _fp64_errors(sub, crop, 7, 3, 1.0, dev, True)}}
+    sc, cams = S.cpu10k()
+    inp = {k: v.detach() for k, v in sc.render_inputs().items()}
+    out["cpu10k"] = {"unsnapped": _fp64_errors(inp, cams[0], 0, 0, 0.0, dev, False)}
+    return out
 
 
 def time_tight(renderer, step, steps):
@@ -531,7 +772,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="full3m")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-div", type=int, default=128)
     ap.add_argument("--batched", type=int, default=1, help="also run the 16-view batched case")
     ap.add_argument("--serve", type=int, default=1, help="also run the serving trace")
     ap.add_argument("--train500k", type=int, default=1,
@@ -639,11 +879,51 @@ def main():
     dom_bytes = ab[dom]
     achieved = dom_bytes / (per_kernel[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = PMC_TRAFFIC_BYTES.get(dom) if args.workload == "full3m" else None
+    copy_gbs = copy_bandwidth()
+    print(f"[bench] device copy {copy_gbs} GB/s", file=sys.stderr, flush=True)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": int(dom_bytes),
-                "traffic_source": "profiles/alpha_blending_bwd_full3m_pmc.md" if traffic else None}
+                "traffic_source": PMC_SOURCE if traffic else None,
+                "copy_measured_GBs": copy_gbs,
+                "frac_of_measured_copy": round(achieved / copy_gbs, 4)}
+    if "sort_count" in per_kernel and "sort_emit" in per_kernel:
+        t_bin = per_kernel["sort_count"]["avg_ms"] + per_kernel["sort_emit"]["avg_ms"]
+        roofline["binning"] = {"ms": round(t_bin, 4), "algorithmic_bytes": int(ab["binning"]),
+                               "achieved": round(ab["binning"] / (t_bin * 1e-3) / 1e9, 1),
+                               "frac": round(ab["binning"] / (t_bin * 1e-3) / 1e9 /
+                                             HBM_PEAK_GBS, 4)}
     print(f"[bench] per-kernel: {json.dumps(per_kernel)}", file=sys.stderr, flush=True)
+
+    # the FP32 C restatement of the whole step on the same inputs: CPU
+    # baseline (timed in full), parity at full size, exact blend work counts
+    cpu = parity = None
+    if args.cpu_baseline:
+        print("[bench] FP32 restatement of the full step on the host ...", file=sys.stderr,
+              flush=True)
+        cpu, parity, pairs = restatement_leg(renderer, scene, cam, targets)
+        ops_ = blend_ops(C, pairs["blended"], pairs["examined"])
+        valu = {}
+        for k in ("alpha_blending_fwd", "alpha_blending_bwd"):
+            if k not in per_kernel:
+                continue
+            t_k = per_kernel[k]["avg_ms"] * 1e-3
+            walked = pairs["examined"] if k.endswith("fwd") else pairs["n_contrib_sum"]
+            valu[k] = {"pairs_blended": pairs["blended"], "pairs_walked": walked,
+                       "gpairs_per_s": round(pairs["blended"] / t_k / 1e9, 1),
+                       "algorithmic_ops": int(ops_[k]),
+                       "achieved_tops": round(ops_[k] / t_k / 1e12, 2),
+                       "peak_tops": round(VALU_PEAK_OPS / 1e12, 2),
+                       "frac": round(ops_[k] / t_k / VALU_PEAK_OPS, 4)}
+            valu[k].update(VALU_PMC.get(k, {}))
+        roofline["valu"] = valu
+        print(f"[bench] cpu baseline {cpu['value']} steps/s; parity {json.dumps(parity)}; "
+              f"valu {json.dumps(valu)}", file=sys.stderr, flush=True)
+        print("[bench] FP64 transcription: cpu10k timing and per-output errors ...",
+              file=sys.stderr, flush=True)
+        cpu["fp64_cpu10k"] = fp64_cpu10k()
+        parity["vs_fp64"] = fp64_errors(scene_cpu, cams[0], dev)
+        print(f"[bench] fp64: {json.dumps(parity['vs_fp64'])}", file=sys.stderr, flush=True)
 
     # the same step with opacity-aware binning (same images, fewer instances;
     # not the msplat tile contract, so reported beside the headline only)
@@ -687,11 +967,6 @@ def main():
         print("[bench] serving ...", file=sys.stderr, flush=True)
         serving = bench_serve(lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
 
-    cpu = None
-    if args.cpu_baseline:
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(wl, args.cpu_div)
-
     line = {
         "metric": "fwd+bwd steps/s @3M Gaussians 1237x822",
         "value": round(1000.0 / ms, 3), "unit": "steps/s", "n_gpus": 1,
@@ -708,6 +983,7 @@ def main():
                    "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in per_kernel.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
         "batched": batched,
         "serving": serving,
         "train500k": train500k,

@@ -144,6 +144,15 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
  * inside a captured graph) is safe: if the device count exceeds it the
  * emission is truncated in bounds and the caller detects it from *m_out. */
 
+/* Exact binning generates the items of its first tile radix pass from the
+ * depth-ordered splats where the pass uses them, instead of writing the
+ * instance list and reading it back; the remaining tile passes and the tile
+ * ranges follow as before.  gsr_set_fused_emission(0) makes it write the
+ * list first (process-wide; for comparisons), 1 restores the default;
+ * returns the previous setting.  Both give the same ids_out and tile_range.
+ * No msplat counterpart. */
+int gsr_set_fused_emission(int enable);
+
 /* Opacity-aware ("tight") binning, no counterpart in msplat: as
  * gsr_sort_count / gsr_sort_emit, but a splat is emitted only into the tiles
  * of its rectangle where it can reach alpha >= 1/255 at some pixel centre

@@ -21,6 +21,7 @@
  * the thread count (the camera gradients are summed per thread in double).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -501,6 +502,62 @@ void ref_blend(const float *uv, const float *conic, const float *opacity, const 
       Tfin[(int64_t)py * W + px] = T;
       ncon[(int64_t)py * W + px] = last;
     }
+}
+
+/* Work counts of ref_blend's walk over the same inputs (nothing is written
+ * but the counts): pairs[0] = (pixel, instance) pairs examined before each
+ * pixel stops (the early-out at T < 1e-4 or the end of its tile's range),
+ * pairs[1] = pairs that pass the alpha >= 1/255 test and are blended (the
+ * forward's contributors; the backward replays exactly these), pairs[2] =
+ * sum of n_contrib (the prefix the backward walks). */
+void ref_blend_pairs(const float *uv, const float *conic, const float *opacity,
+                     const int32_t *ids, const int32_t *ranges, int W, int H, int tw, int th,
+                     int64_t *pairs) {
+  const int gx = (W + tw - 1) / tw;
+  const float power_cut = -5.6f, alpha_max = 0.99f, alpha_min = 1.0f / 255.0f, t_min = 1e-4f;
+  int64_t examined = 0, blended = 0, prefix = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : examined, blended, prefix)
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      const int t = (py / th) * gx + (px / tw);
+      const int s = ranges[2 * t], e = ranges[2 * t + 1];
+      const float fpx = (float)px, fpy = (float)py;
+      float T = 1.0f;
+      int last = 0, j;
+      for (j = s; j < e; ++j) {
+        const int g = ids[j];
+        const float ha = -0.5f * conic[3 * g], nb = -conic[3 * g + 1],
+                    hc = -0.5f * conic[3 * g + 2], o = opacity[g];
+        const float dx = uv[2 * g] - fpx, dy = uv[2 * g + 1] - fpy;
+        const float power = fmaf(ha * dx, dx, fmaf(nb * dx, dy, (hc * dy) * dy));
+        float alpha, test_T;
+        ++examined;
+        if (power > 0.0f) continue;
+        if (power < power_cut && o <= 1.0f) continue;
+        alpha = fminf(alpha_max, o * exp_neg(power));
+        if (alpha < alpha_min) continue;
+        test_T = T * (1.0f - alpha);
+        if (test_T < t_min) break;
+        T = test_T;
+        ++blended;
+        last = j - s + 1;
+      }
+      prefix += last;
+    }
+  pairs[0] = examined;
+  pairs[1] = blended;
+  pairs[2] = prefix;
+}
+
+/* OpenMP threads the parallel loops above use. */
+int ref_num_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
 }
 
 /* ---- fused preprocess (direction, SH colour, project, cov3d, EWA, feature

@@ -39,6 +39,8 @@ def lib():
         L.ref_sort_tight.restype = I64
         L.ref_sort_tight.argtypes = [P, P, P, P, P, I64, I, I, I, I, P, P]
         L.ref_blend.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P]
+        L.ref_blend_pairs.argtypes = [P, P, P, P, P, I, I, I, I, P]
+        L.ref_num_threads.restype = I
         L.ref_preprocess.argtypes = [P, P, P, P, I64, I, I, P, P, P, I, I, F, F, I, P, P, P, P, P,
                                      P]
         L.ref_blend_bwd.argtypes = [P, P, P, P, I, P, P, F, I, I, I, I, P, P, P, P, I64, I64, P,
@@ -156,6 +158,25 @@ def alpha_blending(uv, conic, opacity, feature, ids, tile_range, bg, W, H, tile_
     lib().ref_blend(_p(u), _p(c), _p(o), _p(f), C, _p(i), _p(r), bg, W, H, tile_w, tile_h, _p(out),
                     _p(T), _p(nc))
     return out, T, nc
+
+
+def blend_pairs(uv, conic, opacity, ids, tile_range, W, H, tile_w=16, tile_h=16):
+    """Work counts of the blend walk -> (examined, blended, sum of n_contrib):
+    (pixel, instance) pairs examined before each pixel stops, pairs blended
+    (alpha >= 1/255, the forward's contributors), and the prefix lengths the
+    backward walks."""
+    u, c = _c(uv, np.float32), _c(conic, np.float32)
+    o = _c(np.asarray(opacity).reshape(-1), np.float32)
+    i = _c(ids, np.int32)
+    r = _c(tile_range, np.int32)
+    out = np.zeros(3, np.int64)
+    lib().ref_blend_pairs(_p(u), _p(c), _p(o), _p(i), _p(r), W, H, tile_w, tile_h, _p(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def num_threads() -> int:
+    """OpenMP threads of the restatement's parallel loops."""
+    return int(lib().ref_num_threads())
 
 
 def preprocess(xyz, scale, rot, shs, intr, extr, center, W, H, degree=-1, flags=0, nearest=0.2,

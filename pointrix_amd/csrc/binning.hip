@@ -271,17 +271,38 @@ __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restri
 // every item at its digit-sorted position in LDS and writes the block out in
 // LDS order, so consecutive lanes write consecutive addresses of a digit's
 // output run instead of scattering single words.
+// Items of a radix block from arrays in memory.
+struct ListItems {
+  const uint32_t *kin;
+  const int32_t *vin;
+  // rows wbase + 64 r + lane of this wave (every load issued before the
+  // ranking, whose wave barriers would otherwise expose one latency per row)
+  __device__ __forceinline__ void load(uint32_t, uint32_t wbase, uint32_t n,
+                                       uint32_t (&key)[kItems], int32_t (&val)[kItems],
+                                       uint32_t *, int32_t *, uint32_t *) const {
+    const int lane = lane_id();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const uint32_t i = wbase + r * 64 + lane;
+      const bool valid = i < n;
+      key[r] = valid ? kin[i] : 0u;
+      val[r] = valid ? vin[i] : 0;
+    }
+  }
+};
+
+template <class Items>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
-    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
+    Items items, uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
     const uint32_t *__restrict__ totals, const uint32_t *__restrict__ kgather,
-    uint32_t *__restrict__ rank_out) {
+    uint32_t *__restrict__ rank_out, const uint2 *__restrict__ rgather, uint2 *__restrict__ rout) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
   __shared__ uint32_t s_key[kTileItems];
   __shared__ int32_t s_val[kTileItems];
+  __shared__ uint32_t s_flag[kThreads];
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
   if (base >= n) return;  // uniform over the block
@@ -289,24 +310,16 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   const int w = threadIdx.x >> 6, lane = lane_id();
 #pragma unroll
   for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][threadIdx.x] = 0;
-  __syncthreads();
   const uint64_t lt = lanemask_lt();
   uint32_t key[kItems];
   int32_t val[kItems];
   uint32_t rank[kItems];
   const uint32_t wbase = base + w * (kTileItems / (kThreads / 64));
-  // every load of the block issued before the first ranking step (the
-  // ranking's wave barriers would otherwise expose one load latency per row),
-  // with this thread's digit total and block offset
+  // this thread's digit total and block offset, requested with the items
   const uint32_t my_total = (int)threadIdx.x < (1 << bits) ? totals[threadIdx.x] : 0u;
   const uint32_t my_hist = hist_scanned[threadIdx.x * nb + blockIdx.x];
-#pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = wbase + r * 64 + lane;
-    const bool valid = i < n;
-    key[r] = valid ? kin[i] : 0u;
-    val[r] = valid ? vin[i] : 0;
-  }
+  items.load(blockIdx.x, wbase, n, key, val, s_key, s_val, s_flag);
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t i = wbase + r * 64 + lane;
@@ -372,6 +385,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       vout[pos] = vv;
       // rank_out: each value's sorted position (the inverse permutation)
       if (rank_out != nullptr) rank_out[vv] = pos;
+      // rout: a second per-item array (8 bytes) carried into the sorted order
+      if (rout != nullptr) rout[pos] = rgather[vv];
     }
   }
 }
@@ -648,10 +663,13 @@ __device__ __forceinline__ EmitWave emit_wave_load(const int32_t *__restrict__ s
   return e;
 }
 
-// Instances [cb, ce) of the wave's range, 64 consecutive ones per round.
-__device__ __forceinline__ void emit_rounds(const EmitWave &e, uint32_t cb, uint32_t ce,
-                                            uint32_t *flag, uint32_t *__restrict__ tkeys,
-                                            int32_t *__restrict__ tvals) {
+// Instances [cb, ce) of the wave's range, 64 consecutive ones per round:
+// every lane of the wave calls f(q, tile, id, valid, single) once per round,
+// q = c0 + lane, valid = q < ce; single = the whole round lies inside one
+// splat (every lane valid, the 64 tiles distinct).
+template <class F>
+__device__ __forceinline__ void gen_rounds(const EmitWave &e, uint32_t cb, uint32_t ce,
+                                           uint32_t *flag, F &f) {
   const int lane = lane_id();
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
@@ -672,8 +690,7 @@ __device__ __forceinline__ void emit_rounds(const EmitWave &e, uint32_t cb, uint
         const int k = (int)(q - joff);
         const int row = (int)(((float)k + 0.5f) * jinv);
         const int col = k - row * w;
-        tkeys[q] = (uint32_t)(jtoff + (y0 + row) * gx + (x0 + col));
-        tvals[q] = jid;
+        f(q, (uint32_t)(jtoff + (y0 + row) * gx + (x0 + col)), jid, true, true);
       }
       c0 -= 64;  // the loop increment moves to the first round not yet written
       continue;
@@ -691,15 +708,35 @@ __device__ __forceinline__ void emit_rounds(const EmitWave &e, uint32_t cb, uint
     const int otoff = shfl_i(e.toff, o), oid = shfl_i(e.id, o);
     const float oinv = shfl_f(e.inv_w, o);
     const uint32_t q = c0 + (uint32_t)lane;
-    if (q < ce) {
-      const int k = (int)(q - ooff);
-      const int w = (int)(owgx & 0xFFFFu), gx = (int)(owgx >> 16);
-      const int row = (int)(((float)k + 0.5f) * oinv);
-      const int col = k - row * w;
-      tkeys[q] = (uint32_t)(otoff + ((int)(oxy >> 16) + row) * gx + ((int)(oxy & 0xFFFFu) + col));
-      tvals[q] = oid;
+    const bool valid = q < ce;
+    const int k = valid ? (int)(q - ooff) : 0;
+    const int w = (int)(owgx & 0xFFFFu), gx = (int)(owgx >> 16);
+    const int row = (int)(((float)k + 0.5f) * oinv);
+    const int col = k - row * w;
+    const uint32_t tile =
+        (uint32_t)(otoff + ((int)(oxy >> 16) + row) * gx + ((int)(oxy & 0xFFFFu) + col));
+    f(q, valid ? tile : 0u, oid, valid, false);
+  }
+}
+
+// The emission's writer: instance q -> (tile key, splat id) at q.
+struct EmitStore {
+  uint32_t *tkeys;
+  int32_t *tvals;
+  __device__ __forceinline__ void operator()(uint32_t q, uint32_t tile, int id, bool valid,
+                                             bool) const {
+    if (valid) {
+      tkeys[q] = tile;
+      tvals[q] = id;
     }
   }
+};
+
+__device__ __forceinline__ void emit_rounds(const EmitWave &e, uint32_t cb, uint32_t ce,
+                                            uint32_t *flag, uint32_t *__restrict__ tkeys,
+                                            int32_t *__restrict__ tvals) {
+  EmitStore st{tkeys, tvals};
+  gen_rounds(e, cb, ce, flag, st);
 }
 
 // A wave writes at most kEmitSpan instances of its range; the rest (the waves
@@ -751,6 +788,164 @@ __global__ __launch_bounds__(kThreads) void emit_rest_kernel(
   const uint32_t cb = item.y, ce = min(item.y + kEmitChunk, e.r1);
   emit_rounds(e, cb, ce, s_flag + (threadIdx.x & ~63u), tkeys, tvals);
 }
+
+// ---- the first tile pass generates its items ---------------------------
+// The first LSD pass over the tile bits does not read an emitted instance
+// list: each 4096-instance radix block is generated where it is used, from
+// the depth-ordered splats (id, instance offset and packed rectangle, the
+// rectangles put in depth order by the depth sort's last pass), exactly as
+// the emission would have written it.  A block's wave w owns instances
+// [base + 1024 w, base + 1024 (w + 1)), i.e. the rows the radix kernels give
+// it, and starts at the splat holding its first instance (a 64-way search
+// over the offsets in the histogram kernel, kept for the scatter).  This
+// removes the emitted list's write and its re-read by the first pass
+// (8 + 12 bytes per instance).
+struct SplatStream {
+  const int32_t *ids;   // depth-sorted splat ids
+  const uint32_t *offs; // instance offset of each (strictly increasing)
+  const uint2 *rects;   // packed rectangle of each, in depth order
+};
+
+// Group of 64 depth-sorted splats [sb, sb + 64) as emit_wave_load gives it,
+// the rectangle read in depth order (no gather through the id); a group at
+// or past nv (a prefetch beyond the last) comes back empty.
+__device__ __forceinline__ EmitWave stream_load(const SplatStream &st, uint32_t nv, const Grid &g,
+                                                uint32_t sb) {
+  EmitWave e{0, 0, 0u, 1u, 0xFFFFFFFFu, 0u, 1.0f, 0u, 0u};
+  if (sb >= nv) return e;  // uniform per wave
+  const uint32_t s = sb + (uint32_t)lane_id();
+  if (s < nv) {
+    e.id = st.ids[s];
+    const uint2 r = st.rects[s];
+    e.off = st.offs[s];
+    const ItemGrid ig = item_grid(g, (uint32_t)e.id);
+    e.toff = ig.toff;
+    e.xy = r.x;
+    const uint32_t rw = r.y & 0xFFFFu, rh = r.y >> 16;
+    const uint32_t w = rw > 1u ? rw : 1u;
+    e.wgx = w | ((uint32_t)ig.gx << 16);
+    e.inv_w = 1.0f / (float)w;
+    e.end = e.off + rw * rh;
+  }
+  const int nsplat = min(64, (int)(nv - sb));
+  e.r0 = (uint32_t)__builtin_amdgcn_readlane((int)e.off, 0);
+  e.r1 = (uint32_t)__builtin_amdgcn_readlane((int)e.end, nsplat - 1);
+  return e;
+}
+
+// Depth-order index of the splat holding instance q: the last s < nv with
+// offs[s] <= q (offsets strictly increase: listed splats own >= 1 instance);
+// a 64-way search, one probe per lane per step.
+__device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ offs, uint32_t nv,
+                                               uint32_t q) {
+  uint32_t lo = 0, hi = nv;
+  const uint32_t lane = (uint32_t)lane_id();
+  while (hi - lo > 1u) {
+    const uint32_t step = (hi - lo + 63u) / 64u;
+    const uint32_t idx = lo + lane * step;
+    const uint64_t b = __ballot(idx < hi && offs[idx] <= q);
+    lo += (uint32_t)(63 - __clzll(b)) * step;  // b != 0: offs[lo] <= q
+    hi = min(lo + step, hi);
+  }
+  return lo;
+}
+
+// Instances [q0, q1) in order, from the splat s0 that holds q0, over
+// consecutive groups of 64 splats; the next group is requested before the
+// current one is walked.
+template <class F>
+__device__ __forceinline__ void gen_range(const SplatStream &st, uint32_t nv, const Grid &g,
+                                          uint32_t s0, uint32_t q0, uint32_t q1, uint32_t *flag,
+                                          F &f) {
+  EmitWave cur = stream_load(st, nv, g, s0);
+  uint32_t c0 = q0;
+  for (uint32_t sb = s0; c0 < q1; sb += 64) {
+    const EmitWave nxt = stream_load(st, nv, g, sb + 64);
+    const uint32_t ce = min(q1, cur.r1);
+    gen_rounds(cur, c0, ce, flag, f);
+    c0 = ce;
+    cur = nxt;
+  }
+}
+
+struct GenDigitCount {
+  uint32_t *h;
+  uint32_t mask;
+  __device__ __forceinline__ void operator()(uint32_t, uint32_t tile, int, bool valid,
+                                             bool) const {
+    if (valid) atomicAdd(&h[tile & mask], 1u);
+  }
+};
+
+struct GenStage {
+  uint32_t *key;
+  int32_t *val;
+  uint32_t base;
+  __device__ __forceinline__ void operator()(uint32_t q, uint32_t tile, int id, bool valid,
+                                             bool) const {
+    if (valid) {
+      key[q - base] = tile;
+      val[q - base] = id;
+    }
+  }
+};
+
+// Histogram of the first tile pass (digit = tile & mask) per radix block, as
+// radix_hist_kernel would count the emitted list; first[4 b + w] = the
+// depth-order index of the splat holding wave w's first instance.
+__global__ __launch_bounds__(kThreads) void gen_hist_kernel(
+    SplatStream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
+    uint32_t cap_m, Grid g, uint32_t mask, uint32_t *__restrict__ hist, uint32_t nb,
+    uint32_t *__restrict__ first) {
+  __shared__ uint32_t s_h[kRadix];
+  __shared__ uint32_t s_flag[kThreads];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t M = count_of(m_ptr, cap_m), nv = count_of(nv_ptr, cap_n);
+  const int w = threadIdx.x >> 6;
+  constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
+  const uint32_t q0 = blockIdx.x * kTileItems + (uint32_t)w * kWaveItems;
+  if (q0 < M) {  // uniform per wave
+    const uint32_t s0 = find_owner(st.offs, nv, q0);
+    if (lane_id() == 0) first[blockIdx.x * (kThreads / 64) + w] = s0;
+    GenDigitCount f{s_h, mask};
+    gen_range(st, nv, g, s0, q0, min(q0 + kWaveItems, M), s_flag + 64 * w, f);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// Items of a radix block generated from the depth-ordered splats (the first
+// tile pass): the wave writes its 1024 instances (tile key, splat id) into
+// its quarter of the block's LDS staging arrays, then reads its rows back.
+struct GenItems {
+  SplatStream st;
+  const uint32_t *nv_ptr;
+  uint32_t cap_n;
+  Grid g;
+  const uint32_t *first;
+  __device__ __forceinline__ void load(uint32_t block, uint32_t wbase, uint32_t n,
+                                       uint32_t (&key)[kItems], int32_t (&val)[kItems],
+                                       uint32_t *s_key, int32_t *s_val, uint32_t *s_flag) const {
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t base = block * kTileItems;
+    constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
+    if (wbase < n) {  // uniform per wave
+      GenStage f{s_key, s_val, base};
+      gen_range(st, count_of(nv_ptr, cap_n), g, first[block * (kThreads / 64) + w], wbase,
+                min(wbase + kWaveItems, n), s_flag + 64 * w, f);
+    }
+    // the rows were written by this wave (any lane): LDS is in order per wave
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const uint32_t i = wbase + r * 64 + lane;
+      const bool valid = i < n;
+      key[r] = valid ? s_key[i - base] : 0u;
+      val[r] = valid ? s_val[i - base] : 0;
+    }
+  }
+};
 
 // Tight emission in splat index order, as the tight count walked them: the
 // depth order would put the large near splats, with a reachability test per
@@ -929,16 +1124,21 @@ void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap, uint32_t *
 inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBits; }
 
 // Stable LSD passes over bits [0, nbits) of 32-bit keys.  Data starts in
-// (keyA, valA) -- or in (keyA, v0) when v0 != nullptr -- and the final result
-// is returned through the pointers.
+// (k_cur, v_cur) and the final result is returned through the pointers.
+// Passes [p0, passes) run (p0 > 0: the earlier passes were done elsewhere and
+// their result is in (k_cur, v_cur)).  The last pass can write gathered keys
+// (final_gather[value]), each value's sorted position (final_rank) and a
+// gathered 8-byte row (final_rgather -> final_rout) instead of / besides the
+// keys.
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
                bool final_keys = true, const uint32_t *final_gather = nullptr,
-               uint32_t *final_rank = nullptr) {
+               uint32_t *final_rank = nullptr, const uint2 *final_rgather = nullptr,
+               uint2 *final_rout = nullptr, int p0 = 0) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
-  for (int p = 0; p < passes; ++p) {
+  for (int p = p0; p < passes; ++p) {
     const int shift = p * per;
     const int bits = min(per, nbits - shift);
     const uint32_t mask = (1u << bits) - 1u;
@@ -946,9 +1146,11 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, b.totals);
     const bool last = p == passes - 1;
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        k_cur, v_cur, (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt,
-        n_ptr, cap, shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr,
-        last ? final_rank : nullptr);
+        ListItems{k_cur, v_cur},
+        (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt, n_ptr, cap,
+        shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr,
+        last ? final_rank : nullptr, last ? final_rgather : nullptr,
+        last ? final_rout : nullptr);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
@@ -963,6 +1165,11 @@ int tile_bits(int T) {
   while ((1 << b) < T) ++b;
   return b;
 }
+
+// Exact binning generates its first tile pass's items from the depth-ordered
+// splats (gen_hist_kernel, radix_scatter_kernel<GenItems>) unless switched
+// off, when the emission writes the instance list and list passes sort it.
+bool g_fused_emission = true;
 
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
@@ -999,7 +1206,14 @@ size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n >
 
 size_t gsr_sort_ws_inst_bytes(int64_t m) {
   const uint64_t c = (uint64_t)(m > 0 ? m : 1);
-  return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8);
+  return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8) +
+         align256((uint64_t)nblk(c) * (kThreads / 64) * 4);
+}
+
+int gsr_set_fused_emission(int enable) {
+  const int prev = g_fused_emission;
+  g_fused_emission = enable != 0;
+  return prev;
 }
 
 }  // extern "C"
@@ -1052,8 +1266,13 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // tight binning also needs each splat's depth rank (its emission walks splats
   // in index order); the exact count kernel's packed rectangles are not used
   // there, so their array holds it
+  // exact binning: the packed rectangles are also put in depth order (into
+  // the masks array, which only tight binning uses) for the generated first
+  // tile pass of sort_emit
   run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt,
-            gr.conic != nullptr ? reinterpret_cast<uint32_t *>(g.rects) : nullptr);
+            gr.conic != nullptr ? reinterpret_cast<uint32_t *>(g.rects) : nullptr,
+            gr.conic == nullptr ? g.rects : nullptr,
+            gr.conic == nullptr ? reinterpret_cast<uint2 *>(g.masks) : nullptr);
   // offsets of each splat's instances, in depth order -> M in scalars[1]
   run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
            s);
@@ -1072,16 +1291,42 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   carve_gauss(ws_gauss, (uint64_t)n, g);
   SortBufs ib;
   char *p = carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
-  uint32_t *n_items;
+  uint32_t *n_items, *first;
   uint2 *items;
   const uint64_t items_cap = emit_items_cap((uint64_t)m);
+  const uint32_t nb = nblk((uint64_t)m);
   p = carve(p, 64, (void **)&n_items);
-  carve(p, items_cap * 8, (void **)&items);
+  p = carve(p, items_cap * 8, (void **)&items);
+  carve(p, (uint64_t)nb * (kThreads / 64) * 4, (void **)&first);
   // tile_range and the emission's queue counter start at zero
   zero2_kernel<<<gsr_blocks(2 * (int64_t)T + 1, kThreads), kThreads, 0, s>>>(
       tile_range, 2 * (int64_t)T, n_items, 1);
   const int nbits = tile_bits(T);
   const int passes = radix_passes(nbits);
+  if (gr.conic == nullptr && passes >= 1 && g_fused_emission) {
+    // first tile pass over generated items (sort_count's last depth pass put
+    // the rectangles in depth order into the masks array), then the rest as
+    // list passes; values alternate so that the last pass writes ids_out
+    int32_t *v_pass0 = (passes % 2 == 1) ? ids_out : ib.valA;
+    int32_t *v_pass1 = (passes % 2 == 1) ? ib.valA : ids_out;
+    const int per = (nbits + passes - 1) / passes;
+    const int bits0 = min(per, nbits);
+    const SplatStream st{g.sb.valA, g.offs, reinterpret_cast<const uint2 *>(g.masks)};
+    gen_hist_kernel<<<nb, kThreads, 0, s>>>(st, g.scalars, (uint32_t)n, g.scalars + 1,
+                                            (uint32_t)m, gr, (1u << bits0) - 1u, ib.hist, nb,
+                                            first);
+    digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, ib.totals);
+    radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
+        GenItems{st, g.scalars, (uint32_t)n, gr, first}, ib.keyA, v_pass0, g.scalars + 1,
+        (uint32_t)m, 0, bits0, ib.hist, nb, ib.totals, nullptr, nullptr, nullptr, nullptr);
+    uint32_t *kc = ib.keyA;
+    int32_t *vc = v_pass0;
+    run_radix(kc, vc, ib.keyB, v_pass1, g.scalars + 1, (uint32_t)m, nbits, ib, s, true, nullptr,
+              nullptr, nullptr, nullptr, 1);
+    ranges_kernel<<<gsr_blocks(((int64_t)m + 3) / 4, kThreads), kThreads, 0, s>>>(
+        kc, g.scalars + 1, (uint32_t)m, tile_range);
+    return gsr_launch_status("sort_emit");
+  }
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
   int32_t *v_first = (passes % 2 == 0) ? ids_out : ib.valA;
   int32_t *v_second = (passes % 2 == 0) ? ib.valA : ids_out;

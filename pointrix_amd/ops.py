@@ -275,6 +275,14 @@ class SortWorkspace:
         return self.m_dev
 
 
+def set_fused_emission(enable: bool) -> bool:
+    """Exact binning generates its first tile pass's items from the
+    depth-ordered splats (the default); ``False`` writes the instance list
+    first and sorts it with list passes.  Both give the same ids and tile
+    ranges.  Process-wide; returns the previous setting."""
+    return bool(_lib.load().gsr_set_fused_emission(int(bool(enable))))
+
+
 _default_ws = SortWorkspace()
 last_instance_count = 0  # M of the most recent sort_gaussian call (diagnostics)
 
