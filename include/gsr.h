@@ -153,6 +153,25 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
  * No msplat counterpart. */
 int gsr_set_fused_emission(int enable);
 
+/* gsr_sort_count + gsr_sort_emit in one call that allocates nothing: no
+ * counterpart in msplat or in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The workspaces are carved from a
+ * file-scope device array of 64 MiB that each GPU the library runs on holds
+ * until the process ends (its address is taken once per device, on the first
+ * call there).  n splats and the instance capacity m_cap of ids_out must fit
+ * it together: gsr_sort_ws_gauss_bytes(n) + gsr_sort_ws_inst_bytes(m_cap) <=
+ * 64 MiB (about 500k splats with 2.5M instances), else GSR_ERR_ARG before
+ * anything is queued.  Capacity semantics as gsr_sort_emit with a fixed m:
+ * *m_out receives the device instance count, and if it exceeds m_cap the
+ * sort covers the first m_cap instances in bounds.  Every call on one GPU
+ * uses that GPU's one array: calls queued on one stream are ordered, calls
+ * that overlap on one GPU (other streams, other host threads) race. */
+int gsr_sort_gaussian_noalloc(const float *uv, const float *depth, const int32_t *radius,
+                              int64_t n, int W, int H, int tile_w, int tile_h, int64_t m_cap,
+                              int32_t *m_out, int32_t *ids_out, int32_t *tile_range,
+                              void *stream);
+
 /* Opacity-aware ("tight") binning, no counterpart in msplat: as
  * gsr_sort_count / gsr_sort_emit, but a splat is emitted only into the tiles
  * of its rectangle where it can reach alpha >= 1/255 at some pixel centre

@@ -16,6 +16,8 @@
 // which keeps the sort stable; the block is reordered by digit in LDS before
 // it is written, so writes go out as runs.  Element counts live on the
 // device so no pass needs a host sync.
+#include <mutex>
+
 #include "gsr_internal.h"
 
 using namespace gsr;
@@ -1233,6 +1235,35 @@ int copy_count_to_host(const int32_t *m_out, int32_t *m_host, void *event, hipSt
   return GSR_OK;
 }
 
+// Scratch of gsr_sort_gaussian_noalloc: one file-scope device array, which
+// every GPU the library runs on holds (its own copy) until the process ends.
+// Host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (taken on the first call on that device and kept);
+// the array's name is not a device address on the host.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+char *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_of_current_device(char **out) {
+  int dev = -1;
+  GSR_REQUIRE(hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDevices,
+              "sort_gaussian_noalloc: no current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      gsr_set_error("sort_gaussian_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return e != hipSuccess ? -(int)e : GSR_ERR_ARG;
+    }
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return GSR_OK;
+}
+
 int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, int64_t n,
                     const Grid &gr, void *ws_gauss, int32_t *m_out, hipStream_t s,
                     int32_t *m_host = nullptr, void *event = nullptr) {
@@ -1406,6 +1437,34 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
                   int32_t *tile_range, void *stream) {
   return gsr_sort_emit_tight(uv, radius, nullptr, nullptr, n, W, H, tile_w, tile_h, ws_gauss, m,
                              ws_inst, ids_out, tile_range, stream);
+}
+
+int gsr_sort_gaussian_noalloc(const float *uv, const float *depth, const int32_t *radius,
+                              int64_t n, int W, int H, int tile_w, int tile_h, int64_t m_cap,
+                              int32_t *m_out, int32_t *ids_out, int32_t *tile_range,
+                              void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && m_cap >= 0 && m_cap < (int64_t)INT32_MAX &&
+                  W > 0 && H > 0 && tile_w > 0 && tile_h > 0 &&
+                  (W + tile_w - 1) / tile_w < 65536 && (H + tile_h - 1) / tile_h < 65536,
+              "sort_gaussian_noalloc: bad sizes");
+  const size_t g_bytes = align256(gauss_ws_bytes((uint64_t)(n > 0 ? n : 1)));
+  const size_t i_bytes = gsr_sort_ws_inst_bytes(m_cap);
+  GSR_REQUIRE(g_bytes + i_bytes <= kScratchBytes,
+              "sort_gaussian_noalloc: %lld splats with an instance capacity of %lld need %zu "
+              "bytes of scratch, more than its %zu",
+              (long long)n, (long long)m_cap, g_bytes + i_bytes, kScratchBytes);
+  char *base = nullptr;
+  {
+    const int rc = scratch_of_current_device(&base);
+    if (rc != GSR_OK) return rc;
+  }
+  const int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+  const Grid gr{gx, gy, tile_w, tile_h, nullptr, 0, W, H, nullptr, nullptr};
+  hipStream_t s = (hipStream_t)stream;
+  const int rc = sort_count_impl(uv, depth, radius, n, gr, base, m_out, s);
+  if (rc != GSR_OK) return rc;
+  return sort_emit_impl(uv, radius, n, gr, gx * gy, base, m_cap, base + g_bytes, ids_out,
+                        tile_range, s);
 }
 
 int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius,
