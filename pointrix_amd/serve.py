@@ -34,7 +34,7 @@ from dataclasses import dataclass
 
 import torch
 
-from . import _lib
+from . import _lib, ops
 from . import scene as S
 from .batched import BatchedRender, InstanceBudgetExceeded, ViewLayout, view_floats
 from .renderer import MsplatRender
@@ -125,8 +125,17 @@ class BatchServer:
                                        sh_degree=sh_degree, tile=tile, batch_depth=batch_depth)
                   for depth in (False, True)}
 
+            # a batch of one view runs through the one-view renderer (the same
+            # images: a view of a batch equals that view rendered alone), which
+            # skips the multi-view tables and queue
+            single = {}
+            for depth in (False, True):
+                r = MsplatRender({"render_depth": depth, "max_sh_degree": sh_degree},
+                                 white_bg=white_bg, device="cuda")
+                r.sh_degree = sh_degree
+                single[depth] = r
             self.slots.append({"stream": torch.cuda.Stream(device=self.dev), "renderers": rs,
-                               "busy": None})
+                               "single": single, "busy": None})
         self.max_views = min(max_views, MAX_VIEWS)
         self.tight_min_views = tight_min_views
         self.max_instances = max(1, key_budget_bytes // instance_bytes())
@@ -168,6 +177,9 @@ class BatchServer:
         ``queue`` and the first half is tried (a single view always runs)."""
         sl = slot or self.free_slot()
         assert sl is not None and sl["busy"] is None, "no free slot"
+        if len(batch) == 1:
+            self._submit_one(batch[0], sl)
+            return
         while True:
             need_depth = any(r.depth for r in batch)
             rd = sl["renderers"][need_depth]
@@ -206,6 +218,24 @@ class BatchServer:
             res.append(d)
         sl["busy"] = _Inflight(batch, res, ev, m, start)
         self.batch_sizes.append(len(batch))
+
+    def _submit_one(self, r: Request, sl) -> None:
+        with torch.cuda.stream(sl["stream"]), torch.no_grad():
+            start = torch.cuda.Event(enable_timing=True)
+            start.record()
+            o = sl["single"][r.depth].render_iter(
+                r.height, r.width, r.extrinsic_matrix.to(self.dev, non_blocking=True),
+                r.intrinsic_params.to(self.dev, non_blocking=True),
+                r.camera_center.to(self.dev, non_blocking=True), **self.inputs)
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record()
+        # the count sort_gaussian read ahead of its depth sort
+        m = ops.last_instance_count
+        self._learn(m, r.pixels)
+        f = o["rendered_features_split"]
+        res = {k: f[k] for k in (("rgb", "depth") if r.depth else ("rgb",))}
+        sl["busy"] = _Inflight([r], [res], ev, m, start)
+        self.batch_sizes.append(1)
 
     def poll(self, wait=False) -> list:
         """-> [(request, result, end event)] of the batches that completed
