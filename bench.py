@@ -169,11 +169,17 @@ def graphed_headline(renderer, scene, cam, loss_fn, steps, warmup, log):
     for _ in range(warmup):
         cs.replay()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        cs.replay()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1000 / steps
+    # replays back to back (no host wait between them); the instance count is
+    # checked once afterwards, and an overflow (the step then re-captured)
+    # repeats the timing
+    for _ in range(2):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cs.replay(check=False)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / steps
+        if not cs.overflowed():
+            break
     ok, worst = True, 0.0
     for _ in range(3):
         cs.replay()
@@ -608,11 +614,14 @@ def bench_batched(steps, warmup, log, n=1_000_000, tune=None):
     for _ in range(warmup):
         gs.replay()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        gs.replay()
-    torch.cuda.synchronize()
-    ms_graph = (time.perf_counter() - t0) * 1000 / steps
+    for _ in range(2):  # back to back, the count checked once afterwards
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gs.replay(check=False)
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) * 1000 / steps
+        if not gs.overflowed():
+            break
     worst = 0.0
     for _ in range(3):   # every replay against eager
         gs.replay()

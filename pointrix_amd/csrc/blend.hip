@@ -33,8 +33,9 @@ __global__ __launch_bounds__(TW *TH) void blend_fwd_kernel(
                             feat, ids, bg, out, Tfin, ncon, sm);
 }
 
-// PP pixels per lane: TW*TH/PP threads, batches of TW*TH/PP splats.
-template <int C, int TW, int TH, int PP>
+// PP pixels per lane: TW*TH/PP threads, batches of TW*TH/PP splats.  BG:
+// whether the pixels carry a background / T_final gradient term.
+template <int C, int TW, int TH, int PP, bool BG>
 __global__ __launch_bounds__(TW *TH / PP) void blend_bwd2_kernel(
     const float *__restrict__ uv, const float *__restrict__ conic,
     const float *__restrict__ opacity, const float *__restrict__ feat,
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(TW *TH / PP) void blend_bwd2_kernel(
   __shared__ BlendLds<C, TW * TH / PP> sm;
   const int tile = blockIdx.x;
   const TileView tv{W, H, gx, tile, 0, (int64_t)H * W, 0};
-  blend_bwd_tile2<C, TW, TH, TW * TH / PP, PP>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv,
+  blend_bwd_tile2<C, TW, TH, TW * TH / PP, PP, BG>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv,
                                                conic, opacity, feat, ids, bg, Tfin, ncon, dout,
                                                dTfin, d_splat, ld, sm, d_uv, d_op);
 }
@@ -81,12 +82,18 @@ int launch_bwd(const float *uv, const float *conic, const float *opacity, const 
 #define GSR_BWD_ARGS                                                                          \
   uv, conic, opacity, feature, ids, tile_range, bg, W, H, gx, T_final, n_contrib, d_out, \
       d_T_final, d_splat, ld, d_uv, d_op
-  if (tw == 16 && th == 16)
-    blend_bwd2_kernel<C, 16, 16, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
-  else if (tw == 8 && th == 32)
-    blend_bwd2_kernel<C, 8, 32, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
+  // without a background and a T_final gradient the pixels' background term is
+  // exactly zero: the kernel without it gives the same results
+  const bool bgt = bg != 0.0f || d_T_final != nullptr;
+  if (tw == 16 && th == 16) {
+    if (bgt)
+      blend_bwd2_kernel<C, 16, 16, 2, true><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
+    else
+      blend_bwd2_kernel<C, 16, 16, 2, false><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
+  } else if (tw == 8 && th == 32)
+    blend_bwd2_kernel<C, 8, 32, 2, true><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else if (tw == 32 && th == 8)
-    blend_bwd2_kernel<C, 32, 8, 2><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
+    blend_bwd2_kernel<C, 32, 8, 2, true><<<nt, 128, 0, s>>>(GSR_BWD_ARGS);
   else
     return GSR_ERR_ARG;
 #undef GSR_BWD_ARGS

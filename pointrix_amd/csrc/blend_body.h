@@ -97,7 +97,10 @@ __device__ __forceinline__ void blend_fwd_tile(
   const int px = tx * TW + (int)threadIdx.x % TW;
   const int py = ty * TH + (int)threadIdx.x / TW;
   const bool inside = px < tv.W && py < tv.H;
-  const float fpx = (float)px, fpy = (float)py;
+  float fpx = (float)px, fpy = (float)py;
+  // kept in registers through the walk (the compiler would otherwise convert
+  // them again on every splat)
+  asm volatile("" : "+v"(fpx), "+v"(fpy));
   const int wave = (int)threadIdx.x >> 6;
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
@@ -109,9 +112,12 @@ __device__ __forceinline__ void blend_fwd_tile(
 #pragma unroll
   for (int c = 0; c < C; ++c) acc[c] = 0.0f;
   int last = 0;
-  bool done = !inside;
+  // the pixel is done (outside the image, or stopped) when nonzero; an int
+  // rather than a bool, so the wave-uniform tests on it compile to scalar
+  // compares of a lane mask
+  int done = inside ? 0 : 1;
   for (int base = start; base < end; base += BD) {
-    if (__syncthreads_count(done) == B) break;
+    if (__syncthreads_count(done != 0) == B) break;
     const int j = base + (int)threadIdx.x;
     unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < end) {
@@ -161,19 +167,19 @@ __device__ __forceinline__ void blend_fwd_tile(
       // wave only branches (uniformly) to skip a splat no active lane needs
       // or to leave once every lane is done.  Decisions are exactly those of
       // the straightforward loop (a lane that skips or stops changes nothing).
-      if (__ballot(!done) == 0ull) break;
+      if (__ballot(done == 0) == 0ull) break;
       const int k = __builtin_amdgcn_readfirstlane((int)sm.list[wave][i]);
       const SplatLds sp = sm.sp[k];
       const float dx = sp.xy.x - fpx, dy = sp.xy.y - fpy;
       const float power = splat_power(sp.co, dx, dy);
-      const bool near = !done && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
+      const bool near = done == 0 && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
       if (__ballot(near) == 0ull) continue;
       const float alpha = fminf(kAlphaMax, sp.co.w * exp_neg(power));
       const bool valid = near && !(alpha < kAlphaMin);
       const float test_T = T * (1.0f - alpha);
       const bool stop = valid && test_T < kTMin;
       const bool contrib = valid && !stop;
-      done = done || stop;
+      done = stop ? 1 : done;
       const float w = contrib ? alpha * T : 0.0f;
 #pragma unroll
       for (int c = 0; c < C; ++c) acc[c] = fmaf(sm.f[c * BD + k], w, acc[c]);
@@ -220,12 +226,31 @@ struct PixBwd {
   float fx, fy;
 };
 
-template <int C, int QP>
-__device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, int local,
+// The pixel's offset to the splat, the exponent and whether the splat can
+// reach alpha >= 1/255 there within the pixel's contributors (the skip tests
+// of the forward).
+struct PixNear {
+  float dx, dy, power;
+  bool near;
+};
+
+template <int C>
+__device__ __forceinline__ PixNear pix_near(const PixBwd<C> &p, const SplatLds &sp, int local) {
+  PixNear r;
+  r.dx = sp.xy.x - p.fx;
+  r.dy = sp.xy.y - p.fy;
+  r.power = splat_power(sp.co, r.dx, r.dy);
+  r.near = local < p.last && !(r.power > 0.0f) && !(r.power < kPowerCut && sp.co.w <= 1.0f);
+  return r;
+}
+
+// BG: the pixels have a background / T_final gradient term (bg != 0 or a
+// T_final gradient); without one the term is exactly zero and is skipped.
+template <int C, int QP, bool BG>
+__device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, const PixNear &pn,
                                              const float *f, int fstride, float (&v)[QP]) {
-  const float dx = sp.xy.x - p.fx, dy = sp.xy.y - p.fy;
-  const float power = splat_power(sp.co, dx, dy);
-  const bool near = local < p.last && !(power > 0.0f) && !(power < kPowerCut && sp.co.w <= 1.0f);
+  const float dx = pn.dx, dy = pn.dy, power = pn.power;
+  const bool near = pn.near;
   // pixels that do not use this splat evaluate exp at 0 (power may be large
   // and positive there, and G feeds a product that must stay finite)
   const float G = exp_neg(near ? power : 0.0f);
@@ -244,7 +269,7 @@ __device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, i
   p.S = contrib ? S_n : p.S;
   p.F = contrib ? fg : p.F;
   float dL_dalpha = (fg - S_n) * Tn;
-  dL_dalpha -= p.T_final * inv_om * p.bg_term;
+  if constexpr (BG) dL_dalpha -= p.T_final * inv_om * p.bg_term;
   p.last_alpha = contrib ? alpha : p.last_alpha;
   // alpha = min(0.99, o G): no gradient where clamped
   const float gsel = (contrib && sp.co.w * G < kAlphaMax) ? dL_dalpha : 0.0f;
@@ -258,7 +283,7 @@ __device__ __forceinline__ bool pix_bwd_step(PixBwd<C> &p, const SplatLds &sp, i
   return contrib;
 }
 
-template <int C, int TW, int TH, int BD, int PP = 2>
+template <int C, int TW, int TH, int BD, int PP = 2, bool BG = true>
 __device__ __forceinline__ void blend_bwd_tile2(
     const TileView &tv, int start, int end, const float *__restrict__ uv,
     const float *__restrict__ conic, const float *__restrict__ opacity,
@@ -287,11 +312,26 @@ __device__ __forceinline__ void blend_bwd_tile2(
   const int by0 = ty * TH, byl = min(ty * TH + TH, tv.H) - 1;
   const int my_q = bfly_index<QP>(lane);
   const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
-  // record column of this lane's quantity: compact rows [da, db, dc,
-  // dfeature...] when both dense arrays are given, else the full row
-  const int rec_col = (d_uv_sep != nullptr && d_op_sep != nullptr)
-                          ? my_q - (my_q < 5 ? 2 : 3)
-                          : my_q;
+  // destination of this lane's quantity for splat id: my_dst[id * my_stride +
+  // my_off].  (du, dv) and dopacity go to their own dense arrays when given
+  // (then the record rows are compact, [da, db, dc, dfeature...]), else to
+  // the full row
+  float *my_dst = d_splat;
+  int my_stride = ld;
+  int64_t my_off;
+  {
+    const bool compact = d_uv_sep != nullptr && d_op_sep != nullptr;
+    my_off = compact ? my_q - (my_q < 5 ? 2 : 3) : my_q;
+    if (my_q < 2 && d_uv_sep != nullptr) {
+      my_dst = d_uv_sep;
+      my_stride = 2;
+      my_off = my_q;
+    } else if (my_q == 5 && d_op_sep != nullptr) {
+      my_dst = d_op_sep;
+      my_stride = 1;
+      my_off = -tv.op_offset;
+    }
+  }
 
   PixBwd<C> pp[PP];
   int tile_last = 0;
@@ -386,14 +426,14 @@ __device__ __forceinline__ void blend_bwd_tile2(
       const int k = __builtin_amdgcn_readfirstlane((int)sm.list[wave][i]);
       const int local = bstart + k - start;
       const SplatLds sp = sm.sp[k];
-      // wave-uniform skip when no lane's pixels can use the splat
+      // wave-uniform skip when no lane's pixels can use the splat (every
+      // pixel's test evaluated, no short circuit: the tests feed the step)
+      PixNear pn[PP];
       bool any = false;
 #pragma unroll
       for (int h = 0; h < PP; ++h) {
-        const float dx = sp.xy.x - pp[h].fx, dy = sp.xy.y - pp[h].fy;
-        const float power = splat_power(sp.co, dx, dy);
-        any = any || (local < pp[h].last && !(power > 0.0f) &&
-                      !(power < kPowerCut && sp.co.w <= 1.0f));
+        pn[h] = pix_near(pp[h], sp, local);
+        any = any | pn[h].near;
       }
       if (__ballot(any) == 0ull) continue;
       float v[QP];
@@ -402,17 +442,10 @@ __device__ __forceinline__ void blend_bwd_tile2(
       bool contrib = false;
 #pragma unroll
       for (int h = 0; h < PP; ++h)
-        contrib = pix_bwd_step<C, QP>(pp[h], sp, local, sm.f + k, BD, v) || contrib;
+        contrib = pix_bwd_step<C, QP, BG>(pp[h], sp, pn[h], sm.f + k, BD, v) | contrib;
       if (__ballot(contrib) == 0ull) continue;  // wave-uniform
       Bfly<QP, 32>::run(v, lane);
-      if (writer) {
-        // (du, dv) and dopacity go to their own dense arrays when given
-        const int64_t id = sm.id[k];
-        float *dst = d_splat + id * ld + rec_col;
-        if (my_q < 2 && d_uv_sep != nullptr) dst = d_uv_sep + 2 * id + my_q;
-        if (my_q == 5 && d_op_sep != nullptr) dst = d_op_sep + (id - tv.op_offset);
-        atomicAdd(dst, v[0]);
-      }
+      if (writer) atomicAdd(my_dst + ((int64_t)sm.id[k] * (int64_t)my_stride + my_off), v[0]);
     }
   }
 }
