@@ -198,17 +198,49 @@ struct StoreU32 {
 // bits <= 8; a sort over nbits key bits uses ceil(nbits / 8) passes of equal
 // width (12 tile bits: two passes of 6), which keeps every pass's buckets as
 // few and as long as the pass count allows.
+// Tile counts gathered by the histogram of the last of two tile passes (its
+// input is sorted by the low digit, the key bits below `shift`): per block b,
+// the items of the block's first low digit (slot 0) and of the next one
+// (slot 1) counted per high digit into tcnt[b][slot][digit]; items of later low
+// digits (a block spanning more than two) add to over[tile] with a global
+// atomic.  tile_count_kernel then sums each tile's entries over the blocks.
+struct TileCount {
+  uint32_t *tcnt;  // [nb][2][kRadix]; nullptr: not gathered
+  uint32_t *over;  // [T], zero on entry
+};
+
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys,
                                                               const uint32_t *n_ptr,
                                                               uint32_t cap, int shift,
                                                               uint32_t mask,
                                                               uint32_t *__restrict__ hist,
-                                                              uint32_t nb) {
+                                                              uint32_t nb, TileCount tc) {
   __shared__ uint32_t s_h[kRadix];
+  __shared__ uint32_t s_t[2][kRadix];
   s_h[threadIdx.x] = 0;
+  s_t[0][threadIdx.x] = 0;
+  s_t[1][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
+  const bool tiles = tc.tcnt != nullptr;
+  const uint32_t lmask = (1u << shift) - 1u;
+  const uint32_t lf = (tiles && base < n) ? (keys[base] & lmask) : 0u;
+  // a block of one low digit (nearly all of them) has its slot-0 counts in
+  // s_h already; only blocks that span several count per slot
+  const bool split =
+      tiles && base < n && (keys[min(base + kTileItems, n) - 1] & lmask) != lf;
+  auto count = [&](uint32_t k) {
+    const uint32_t d = (k >> shift) & mask;
+    atomicAdd(&s_h[d], 1u);
+    if (split) {
+      const uint32_t slot = (k & lmask) - lf;
+      if (slot < 2u)
+        atomicAdd(&s_t[slot][d], 1u);
+      else
+        atomicAdd(&tc.over[k], 1u);
+    }
+  };
   if (base + kTileItems <= n) {
     // full block: 16-byte loads, all in flight before the LDS counts
     // (keys and the block base are 16-byte aligned)
@@ -218,20 +250,123 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
       v[k] = reinterpret_cast<const uint4 *>(keys + base)[k * kThreads + threadIdx.x];
 #pragma unroll
     for (int k = 0; k < kItems / 4; ++k) {
-      atomicAdd(&s_h[(v[k].x >> shift) & mask], 1u);
-      atomicAdd(&s_h[(v[k].y >> shift) & mask], 1u);
-      atomicAdd(&s_h[(v[k].z >> shift) & mask], 1u);
-      atomicAdd(&s_h[(v[k].w >> shift) & mask], 1u);
+      count(v[k].x);
+      count(v[k].y);
+      count(v[k].z);
+      count(v[k].w);
     }
   } else if (base < n) {
 #pragma unroll 4
     for (int k = 0; k < kItems; ++k) {
       const uint32_t i = base + k * kThreads + threadIdx.x;
-      if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
+      if (i < n) count(keys[i]);
     }
   }
   __syncthreads();
   hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+  if (tiles) {
+    tc.tcnt[(2 * (size_t)blockIdx.x) * kRadix + threadIdx.x] =
+        split ? s_t[0][threadIdx.x] : s_h[threadIdx.x];
+    tc.tcnt[(2 * (size_t)blockIdx.x + 1) * kRadix + threadIdx.x] =
+        split ? s_t[1][threadIdx.x] : 0u;
+  }
+}
+
+// counts[t] for tile t = (high digit h, low digit l).  The first pass's digit
+// totals give the low digits' starts P_l in its output, so block b (items
+// [4096 b, 4096 b + 4096)) starts in digit l exactly when ceil(P_l / 4096) <= b
+// < ceil(P_{l+1} / 4096): the blocks holding digit l in slot 1 (first digit
+// l - 1) and slot 0 (first digit l) are two known runs, and count[t] is the sum
+// of their entries plus over[t].  One workgroup per low digit; thread (g, h),
+// h the high digit (hbits bits) and g one of the 256 >> hbits groups, sums
+// every group-th block of the runs (independent loads, eight in flight); the
+// groups are added in LDS.
+__global__ __launch_bounds__(kThreads) void tile_count_kernel(
+    const uint32_t *__restrict__ tcnt, const uint32_t *__restrict__ totals1, int lbits,
+    const uint32_t *__restrict__ over, int hbits, int T, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_c[kThreads];
+  __shared__ uint32_t s_b[3];
+  const uint32_t l = blockIdx.x;
+  {
+    // P_{l-1}, P_l, P_{l+1}: prefix sums of the first pass's totals
+    const uint32_t d = threadIdx.x;
+    const uint32_t v = d < (1u << lbits) ? totals1[d] : 0u;
+    uint32_t tot;
+    const uint32_t excl = block_exclusive_scan(v, tot);
+    const uint32_t blk = (excl + kTileItems - 1) / kTileItems;  // ceil(P_d / 4096)
+    if (l > 0u && d == l - 1u) s_b[0] = blk;
+    if (d == l) s_b[1] = blk;
+    if (d == l + 1u) s_b[2] = blk;
+    if (l == 0u && d == 0u) s_b[0] = blk;
+    if (l + 1u == (1u << lbits) && d == 0u) s_b[2] = (tot + kTileItems - 1) / kTileItems;
+  }
+  __syncthreads();
+  const uint32_t h = threadIdx.x & ((1u << hbits) - 1u), grp = threadIdx.x >> hbits;
+  const uint32_t ngrp = (uint32_t)kThreads >> hbits;
+  const uint32_t b0 = s_b[0], b1 = s_b[1], b2 = s_b[2];
+  uint32_t c = 0;
+  for (uint32_t b = b0 + grp; b < b2; b += 8 * ngrp) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t bb = b + k * ngrp;
+      // blocks [b0, b1) start in digit l - 1 (slot 1), [b1, b2) in digit l
+      v[k] = bb < b2 ? tcnt[(2 * (size_t)bb + (bb < b1 ? 1u : 0u)) * kRadix + h] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c += v[k];
+  }
+  s_c[threadIdx.x] = c;
+  __syncthreads();
+  const int t = (int)((h << lbits) | l);
+  if (grp == 0 && t < T) {
+    for (uint32_t q = 1; q < ngrp; ++q) c += s_c[(q << hbits) | h];
+    counts[t] = c + over[t];
+  }
+}
+
+// tile_range[t] = [start, start + counts[t]) with start the exclusive prefix of
+// the counts, (0, 0) for an empty tile; one workgroup, 4096 tiles per round.
+__global__ __launch_bounds__(1024) void ranges_from_counts_kernel(
+    const uint32_t *__restrict__ counts, int T, int32_t *__restrict__ tile_range) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int r0 = 0; r0 < T; r0 += 4096) {
+    const int t0 = r0 + 4 * threadIdx.x;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = t0 + j < T ? counts[t0 + j] : 0u;
+      sum += v[j];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += u;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t run = s_carry + inc - sum, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      if (k < w) run += s_w[k];
+      tot += s_w[k];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (t0 + j < T) {
+        tile_range[2 * (t0 + j)] = v[j] ? (int32_t)run : 0;
+        tile_range[2 * (t0 + j) + 1] = v[j] ? (int32_t)(run + v[j]) : 0;
+      }
+      run += v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
 }
 
 // Per digit d (one block each): hist[d][b] <- sum of hist[d][b'] over
@@ -243,7 +378,26 @@ __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restri
   uint32_t *row = hist + (size_t)blockIdx.x * nb;
   const uint32_t per = (nb + kThreads - 1) / kThreads;
   const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
-  // the run is read in groups of 8 independent loads (one latency per group)
+  constexpr uint32_t kHeld = 32;
+  if (per <= kHeld) {
+    // the run held in registers: one read (all loads in flight), one write
+    uint32_t v[kHeld], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      v[k] = b0 + k < b1 ? row[b0 + k] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kHeld; ++k) {
+      if (b0 + k < b1) row[b0 + k] = run;
+      run += v[k];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+    return;
+  }
+  // longer runs are read in groups of 8 independent loads (one latency per group)
   uint32_t sum = 0;
   for (uint32_t b = b0; b < b1; b += 8) {
     uint32_t v[8];
@@ -1136,7 +1290,8 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
                bool final_keys = true, const uint32_t *final_gather = nullptr,
                uint32_t *final_rank = nullptr, const uint2 *final_rgather = nullptr,
-               uint2 *final_rout = nullptr, int p0 = 0) {
+               uint2 *final_rout = nullptr, int p0 = 0, TileCount final_tc = TileCount{},
+               uint32_t *totals0 = nullptr) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
@@ -1144,13 +1299,16 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     const int shift = p * per;
     const int bits = min(per, nbits - shift);
     const uint32_t mask = (1u << bits) - 1u;
-    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb);
-    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, b.totals);
     const bool last = p == passes - 1;
+    // totals0: the first pass's digit totals kept apart (later passes reuse b.totals)
+    uint32_t *totals = (p == 0 && totals0 != nullptr) ? totals0 : b.totals;
+    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb,
+                                              last ? final_tc : TileCount{});
+    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         ListItems{k_cur, v_cur},
         (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt, n_ptr, cap,
-        shift, bits, b.hist, nb, b.totals, last ? final_gather : nullptr,
+        shift, bits, b.hist, nb, totals, last ? final_gather : nullptr,
         last ? final_rank : nullptr, last ? final_rgather : nullptr,
         last ? final_rout : nullptr);
     uint32_t *kt = k_cur;
@@ -1167,6 +1325,11 @@ int tile_bits(int T) {
   while ((1 << b) < T) ++b;
   return b;
 }
+
+// Tile grids of at most kCountTiles tiles (two tile passes or fewer) get their
+// tile ranges from counts gathered by the last pass's histogram, so that pass
+// writes no keys and no kernel re-reads them.
+constexpr int kCountTiles = 1 << 16;
 
 // Exact binning generates its first tile pass's items from the depth-ordered
 // splats (gen_hist_kernel, radix_scatter_kernel<GenItems>) unless switched
@@ -1208,8 +1371,10 @@ size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n >
 
 size_t gsr_sort_ws_inst_bytes(int64_t m) {
   const uint64_t c = (uint64_t)(m > 0 ? m : 1);
+  const uint64_t nb = nblk(c);
   return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8) +
-         align256((uint64_t)nblk(c) * (kThreads / 64) * 4);
+         align256(nb * (kThreads / 64) * 4) + align256(nb * 2 * kRadix * 4) +
+         align256(kRadix * 4) + 2 * align256((uint64_t)kCountTiles * 4);
 }
 
 int gsr_set_fused_emission(int enable) {
@@ -1322,40 +1487,69 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   carve_gauss(ws_gauss, (uint64_t)n, g);
   SortBufs ib;
   char *p = carve_sortbufs((char *)ws_inst, (uint64_t)m, ib);
-  uint32_t *n_items, *first;
+  uint32_t *n_items, *first, *totals1, *over, *counts;
+  uint32_t *tcnt;
   uint2 *items;
   const uint64_t items_cap = emit_items_cap((uint64_t)m);
   const uint32_t nb = nblk((uint64_t)m);
   p = carve(p, 64, (void **)&n_items);
   p = carve(p, items_cap * 8, (void **)&items);
-  carve(p, (uint64_t)nb * (kThreads / 64) * 4, (void **)&first);
-  // tile_range and the emission's queue counter start at zero
-  zero2_kernel<<<gsr_blocks(2 * (int64_t)T + 1, kThreads), kThreads, 0, s>>>(
-      tile_range, 2 * (int64_t)T, n_items, 1);
+  p = carve(p, (uint64_t)nb * (kThreads / 64) * 4, (void **)&first);
+  p = carve(p, (uint64_t)nb * 2 * kRadix * 4, (void **)&tcnt);
+  p = carve(p, (uint64_t)kRadix * 4, (void **)&totals1);
+  p = carve(p, (uint64_t)kCountTiles * 4, (void **)&over);
+  carve(p, (uint64_t)kCountTiles * 4, (void **)&counts);
   const int nbits = tile_bits(T);
   const int passes = radix_passes(nbits);
+  // ranges from counts: one pass (its digit totals are the tile counts) or two
+  // (counts gathered by the last histogram); more passes, or one tile (no
+  // pass), take them from the sorted keys
+  const bool by_counts = T <= kCountTiles && (passes == 1 || passes == 2);
+  const bool by_keys = !by_counts;
+  // the emission's queue counter starts at zero; so do tile_range when it is
+  // filled from the keys, and the spill counts of the last histogram
+  zero2_kernel<<<gsr_blocks((by_keys ? 2 * (int64_t)T : 0) + 1, kThreads), kThreads, 0, s>>>(
+      tile_range, by_keys ? 2 * (int64_t)T : 0, n_items, 1);
+  if (passes == 2 && by_counts)
+    fill_kernel<<<gsr_blocks(T, kThreads), kThreads, 0, s>>>(reinterpret_cast<int32_t *>(over),
+                                                            T, 0);
+  const int per = passes ? (nbits + passes - 1) / passes : 0;
+  const TileCount tc = (passes == 2 && by_counts) ? TileCount{tcnt, over} : TileCount{};
+  auto ranges = [&](uint32_t *keys_final) {
+    if (by_keys) {
+      ranges_kernel<<<gsr_blocks(((int64_t)m + 3) / 4, kThreads), kThreads, 0, s>>>(
+          keys_final, g.scalars + 1, (uint32_t)m, tile_range);
+      return;
+    }
+    const uint32_t *cnt = totals1;  // one pass: its digit totals
+    if (passes == 2) {
+      tile_count_kernel<<<1u << per, kThreads, 0, s>>>(tcnt, totals1, per, over, nbits - per,
+                                                        T, counts);
+      cnt = counts;
+    }
+    ranges_from_counts_kernel<<<1, 1024, 0, s>>>(cnt, T, tile_range);
+  };
   if (gr.conic == nullptr && passes >= 1 && g_fused_emission) {
     // first tile pass over generated items (sort_count's last depth pass put
     // the rectangles in depth order into the masks array), then the rest as
     // list passes; values alternate so that the last pass writes ids_out
     int32_t *v_pass0 = (passes % 2 == 1) ? ids_out : ib.valA;
     int32_t *v_pass1 = (passes % 2 == 1) ? ib.valA : ids_out;
-    const int per = (nbits + passes - 1) / passes;
     const int bits0 = min(per, nbits);
     const SplatStream st{g.sb.valA, g.offs, reinterpret_cast<const uint2 *>(g.masks)};
     gen_hist_kernel<<<nb, kThreads, 0, s>>>(st, g.scalars, (uint32_t)n, g.scalars + 1,
                                             (uint32_t)m, gr, (1u << bits0) - 1u, ib.hist, nb,
                                             first);
-    digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, ib.totals);
+    digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, totals1);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        GenItems{st, g.scalars, (uint32_t)n, gr, first}, ib.keyA, v_pass0, g.scalars + 1,
-        (uint32_t)m, 0, bits0, ib.hist, nb, ib.totals, nullptr, nullptr, nullptr, nullptr);
+        GenItems{st, g.scalars, (uint32_t)n, gr, first},
+        (passes == 1 && by_counts) ? nullptr : ib.keyA, v_pass0, g.scalars + 1, (uint32_t)m, 0,
+        bits0, ib.hist, nb, totals1, nullptr, nullptr, nullptr, nullptr);
     uint32_t *kc = ib.keyA;
     int32_t *vc = v_pass0;
-    run_radix(kc, vc, ib.keyB, v_pass1, g.scalars + 1, (uint32_t)m, nbits, ib, s, true, nullptr,
-              nullptr, nullptr, nullptr, 1);
-    ranges_kernel<<<gsr_blocks(((int64_t)m + 3) / 4, kThreads), kThreads, 0, s>>>(
-        kc, g.scalars + 1, (uint32_t)m, tile_range);
+    run_radix(kc, vc, ib.keyB, v_pass1, g.scalars + 1, (uint32_t)m, nbits, ib, s, by_keys,
+              nullptr, nullptr, nullptr, nullptr, 1, tc);
+    ranges(kc);
     return gsr_launch_status("sort_emit");
   }
   // depth-sorted splat ids are where sort_count's 4 passes left them: keyA/valA
@@ -1374,9 +1568,9 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
         (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
-  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s);
-  ranges_kernel<<<gsr_blocks(((int64_t)m + 3) / 4, kThreads), kThreads, 0, s>>>(
-      kc, g.scalars + 1, (uint32_t)m, tile_range);
+  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s, by_keys,
+            nullptr, nullptr, nullptr, nullptr, 0, tc, totals1);
+  ranges(kc);
   return gsr_launch_status("sort_emit");
 }
 
