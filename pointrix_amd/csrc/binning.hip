@@ -1024,15 +1024,6 @@ __device__ __forceinline__ void gen_range(const SplatStream &st, uint32_t nv, co
   }
 }
 
-struct GenDigitCount {
-  uint32_t *h;
-  uint32_t mask;
-  __device__ __forceinline__ void operator()(uint32_t, uint32_t tile, int, bool valid,
-                                             bool) const {
-    if (valid) atomicAdd(&h[tile & mask], 1u);
-  }
-};
-
 struct GenStage {
   uint32_t *key;
   int32_t *val;
@@ -1046,29 +1037,82 @@ struct GenStage {
   }
 };
 
-// Histogram of the first tile pass (digit = tile & mask) per radix block, as
-// radix_hist_kernel would count the emitted list; first[4 b + w] = the
+// Histogram of the first tile pass (digit = tile & mask, D = mask + 1 a power
+// of two <= 256) per radix block, as radix_hist_kernel would count the emitted
+// list, without generating the instances: a splat's instances in one row of
+// its rectangle are the consecutive tiles t0 .. t0 + len - 1, whose digits are
+// len / D full cycles plus a run of len % D digits from t0 & mask, wrapping.
+// Each lane takes one splat of the wave's current 64 and adds, for each of its
+// rectangle rows inside the wave's instance range (the first and last rows may
+// be partial), the full cycles to one counter and the run to a difference
+// array over the digits (+1 at its start, -1 past its end, split at the wrap);
+// a block scan over the digits then gives the counts.  first[4 b + w] = the
 // depth-order index of the splat holding wave w's first instance.
 __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     SplatStream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
     uint32_t cap_m, Grid g, uint32_t mask, uint32_t *__restrict__ hist, uint32_t nb,
     uint32_t *__restrict__ first) {
-  __shared__ uint32_t s_h[kRadix];
-  __shared__ uint32_t s_flag[kThreads];
-  s_h[threadIdx.x] = 0;
+  __shared__ int32_t s_d[kRadix + 1];
+  __shared__ uint32_t s_full;
+  s_d[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    s_d[kRadix] = 0;
+    s_full = 0;
+  }
   __syncthreads();
   const uint32_t M = count_of(m_ptr, cap_m), nv = count_of(nv_ptr, cap_n);
   const int w = threadIdx.x >> 6;
+  const uint32_t D = mask + 1u;
+  const uint32_t dbits = (uint32_t)__popc(mask);
   constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
   const uint32_t q0 = blockIdx.x * kTileItems + (uint32_t)w * kWaveItems;
   if (q0 < M) {  // uniform per wave
+    const uint32_t q1 = min(q0 + kWaveItems, M);
     const uint32_t s0 = find_owner(st.offs, nv, q0);
     if (lane_id() == 0) first[blockIdx.x * (kThreads / 64) + w] = s0;
-    GenDigitCount f{s_h, mask};
-    gen_range(st, nv, g, s0, q0, min(q0 + kWaveItems, M), s_flag + 64 * w, f);
+    EmitWave cur = stream_load(st, nv, g, s0);
+    for (uint32_t sb = s0;; sb += 64) {
+      // the next group is requested before this one is counted
+      const EmitWave nxt = stream_load(st, nv, g, sb + 64);
+      // this lane's splat: its instances inside [q0, q1), relative to its
+      // first instance, row by row of its rectangle
+      const uint32_t a = max(cur.off, q0), b = min(cur.end, q1);
+      if (cur.off != 0xFFFFFFFFu && a < b) {
+        const int rw = (int)(cur.wgx & 0xFFFFu), gx = (int)(cur.wgx >> 16);
+        const int x0 = (int)(cur.xy & 0xFFFFu), y0 = (int)(cur.xy >> 16);
+        const int k0 = (int)(a - cur.off), k1 = (int)(b - cur.off) - 1;
+        // row of instance k: trunc((k + 0.5) / w) in float, exact here
+        const int r0 = (int)(((float)k0 + 0.5f) * cur.inv_w);
+        const int r1 = (int)(((float)k1 + 0.5f) * cur.inv_w);
+        const int c0 = k0 - r0 * rw, c1 = k1 - r1 * rw;
+        for (int r = r0; r <= r1; ++r) {
+          const int cs = r == r0 ? c0 : 0, ce = r == r1 ? c1 : rw - 1;
+          const uint32_t len = (uint32_t)(ce - cs + 1);
+          const uint32_t t0 = (uint32_t)(cur.toff + (y0 + r) * gx + x0 + cs);
+          const uint32_t full = len >> dbits, rem = len & mask;
+          if (full) atomicAdd(&s_full, full);
+          if (rem) {
+            const uint32_t d0 = t0 & mask, d1 = d0 + rem;
+            atomicAdd(&s_d[d0], 1);
+            if (d1 <= D) {
+              atomicAdd(&s_d[d1], -1);
+            } else {  // wraps past the last digit
+              atomicAdd(&s_d[0], 1);
+              atomicAdd(&s_d[d1 - D], -1);
+            }
+          }
+        }
+      }
+      if (cur.r1 >= q1 || sb + 64 >= nv) break;  // uniform
+      cur = nxt;
+    }
   }
   __syncthreads();
-  hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+  // counts = the cycles every digit got + the prefix of the difference array
+  const int32_t v = threadIdx.x < D ? s_d[threadIdx.x] : 0;
+  uint32_t total;
+  const uint32_t excl = block_exclusive_scan((uint32_t)v, total);
+  hist[threadIdx.x * nb + blockIdx.x] = threadIdx.x < D ? s_full + excl + (uint32_t)v : 0u;
 }
 
 // Items of a radix block generated from the depth-ordered splats (the first
