@@ -447,17 +447,33 @@ struct ListItems {
   }
 };
 
-template <class Items>
+// What a scatter pass writes besides the sorted values (all optional):
+// kgather: the key output receives kgather[value] (a per-item array carried
+// into the sorted order) instead of the key; rank_out: each value's sorted
+// position (the inverse permutation); pin / pout: an 8-byte payload per item
+// (pin in the input order) moved with it; kout_area: the key output receives
+// the payload's packed rectangle area (w * h of (x0 | y0 << 16, w | h << 16)).
+struct ScatterExtras {
+  const uint32_t *kgather = nullptr;
+  uint32_t *rank_out = nullptr;
+  const uint2 *pin = nullptr;
+  uint2 *pout = nullptr;
+  bool kout_area = false;
+};
+
+// kPay: the pass carries a payload (ex.pin / ex.pout); its LDS index array
+// exists only then, so the other passes keep their occupancy.
+template <class Items, bool kPay = false>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     Items items, uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
-    const uint32_t *__restrict__ totals, const uint32_t *__restrict__ kgather,
-    uint32_t *__restrict__ rank_out, const uint2 *__restrict__ rgather, uint2 *__restrict__ rout) {
+    const uint32_t *__restrict__ totals, ScatterExtras ex) {
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
   __shared__ uint32_t s_key[kTileItems];
   __shared__ int32_t s_val[kTileItems];
+  __shared__ uint16_t s_src[kPay ? kTileItems : 1];  // block-local input index
   __shared__ uint32_t s_flag[kThreads];
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
@@ -524,6 +540,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const uint32_t lp = s_cnt[w][(key[r] >> shift) & mask] + rank[r];
       s_key[lp] = key[r];
       s_val[lp] = val[r];
+      if constexpr (kPay) s_src[lp] = (uint16_t)(i - base);
     }
   }
   __syncthreads();
@@ -535,14 +552,17 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const uint32_t kk = s_key[i];
       const uint32_t pos = i + (uint32_t)s_delta[(kk >> shift) & mask];
       const int32_t vv = s_val[i];
-      // kgather: the key output receives kgather[value] (a per-item array
-      // carried into the sorted order) instead of the key
-      if (kout != nullptr) kout[pos] = kgather != nullptr ? kgather[vv] : kk;
+      uint2 pl = make_uint2(0u, 0u);
+      if constexpr (kPay) {
+        // the payload row of this item, read from the block's input rows
+        pl = ex.pin[base + s_src[i]];
+        ex.pout[pos] = pl;
+      }
+      if (kout != nullptr)
+        kout[pos] = ex.kout_area ? (pl.y & 0xFFFFu) * (pl.y >> 16)
+                                 : (ex.kgather != nullptr ? ex.kgather[vv] : kk);
       vout[pos] = vv;
-      // rank_out: each value's sorted position (the inverse permutation)
-      if (rank_out != nullptr) rank_out[vv] = pos;
-      // rout: a second per-item array (8 bytes) carried into the sorted order
-      if (rout != nullptr) rout[pos] = rgather[vv];
+      if (ex.rank_out != nullptr) ex.rank_out[vv] = pos;
     }
   }
 }
@@ -758,14 +778,20 @@ struct LoadFlag {
   const uint32_t *cnt;
   __device__ uint32_t operator()(uint32_t i) const { return cnt[i] > 0 ? 1u : 0u; }
 };
+// Compaction of the splats that touch a tile: depth bits as keys, the splat
+// index as values, and (exact binning) the packed rectangle as the depth
+// sort's payload.
 struct StoreCompact {
   const float *depth;
   uint32_t *keys;
   int32_t *vals;
+  const uint2 *rin;  // nullptr: no payload
+  uint2 *rout;
   __device__ void operator()(uint32_t i, uint32_t excl, uint32_t flag) const {
     if (flag) {
       keys[excl] = __float_as_uint(depth[i]);
       vals[excl] = (int32_t)i;
+      if (rin != nullptr) rout[excl] = rin[i];
     }
   }
 };
@@ -1326,41 +1352,63 @@ inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBit
 // Stable LSD passes over bits [0, nbits) of 32-bit keys.  Data starts in
 // (k_cur, v_cur) and the final result is returned through the pointers.
 // Passes [p0, passes) run (p0 > 0: the earlier passes were done elsewhere and
-// their result is in (k_cur, v_cur)).  The last pass can write gathered keys
-// (final_gather[value]), each value's sorted position (final_rank) and a
-// gathered 8-byte row (final_rgather -> final_rout) instead of / besides the
-// keys.
+// their result is in (k_cur, v_cur)).  An 8-byte payload per item (pay_cur,
+// ping-ponged with pay_alt; nullptr for none) moves with the items.  The last
+// pass can write, instead of the keys, gathered keys (final_gather[value]) or
+// the payload's rectangle area (final_area), and each value's sorted position
+// (final_rank); final_tc gathers its tile counts.
+struct RadixOpts {
+  bool final_keys = true;
+  const uint32_t *final_gather = nullptr;
+  uint32_t *final_rank = nullptr;
+  uint2 *pay_cur = nullptr, *pay_alt = nullptr;
+  bool final_area = false;
+  int p0 = 0;
+  TileCount final_tc = TileCount{};
+  uint32_t *totals0 = nullptr;  // the first pass's digit totals, kept apart
+};
+
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
-               bool final_keys = true, const uint32_t *final_gather = nullptr,
-               uint32_t *final_rank = nullptr, const uint2 *final_rgather = nullptr,
-               uint2 *final_rout = nullptr, int p0 = 0, TileCount final_tc = TileCount{},
-               uint32_t *totals0 = nullptr) {
+               RadixOpts o = RadixOpts{}) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
-  for (int p = p0; p < passes; ++p) {
+  for (int p = o.p0; p < passes; ++p) {
     const int shift = p * per;
     const int bits = min(per, nbits - shift);
     const uint32_t mask = (1u << bits) - 1u;
     const bool last = p == passes - 1;
-    // totals0: the first pass's digit totals kept apart (later passes reuse b.totals)
-    uint32_t *totals = (p == 0 && totals0 != nullptr) ? totals0 : b.totals;
+    uint32_t *totals = (p == 0 && o.totals0 != nullptr) ? o.totals0 : b.totals;
     radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb,
-                                              last ? final_tc : TileCount{});
+                                              last ? o.final_tc : TileCount{});
     digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals);
-    radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        ListItems{k_cur, v_cur},
-        (last && !final_keys && final_gather == nullptr) ? nullptr : k_alt, v_alt, n_ptr, cap,
-        shift, bits, b.hist, nb, totals, last ? final_gather : nullptr,
-        last ? final_rank : nullptr, last ? final_rgather : nullptr,
-        last ? final_rout : nullptr);
+    ScatterExtras ex;
+    ex.pin = o.pay_cur;
+    ex.pout = o.pay_alt;
+    if (last) {
+      ex.kgather = o.final_gather;
+      ex.rank_out = o.final_rank;
+      ex.kout_area = o.final_area;
+    }
+    const bool keys_out = !last || o.final_keys || o.final_gather != nullptr || o.final_area;
+    if (ex.pin != nullptr)
+      radix_scatter_kernel<ListItems, true><<<nb, kThreads, 0, s>>>(
+          ListItems{k_cur, v_cur}, keys_out ? k_alt : nullptr, v_alt, n_ptr, cap, shift, bits,
+          b.hist, nb, totals, ex);
+    else
+      radix_scatter_kernel<ListItems, false><<<nb, kThreads, 0, s>>>(
+          ListItems{k_cur, v_cur}, keys_out ? k_alt : nullptr, v_alt, n_ptr, cap, shift, bits,
+          b.hist, nb, totals, ex);
     uint32_t *kt = k_cur;
     k_cur = k_alt;
     k_alt = kt;
     int32_t *vt = v_cur;
     v_cur = v_alt;
     v_alt = vt;
+    uint2 *pt = o.pay_cur;
+    o.pay_cur = o.pay_alt;
+    o.pay_alt = pt;
   }
 }
 
@@ -1386,13 +1434,14 @@ struct GaussWs {
   uint32_t *bsum;                  // instances per count block
   uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
   uint2 *rects;                    // exact binning: packed tile rectangles
+  uint2 *rects2;                   // their ping-pong partner in the depth sort
   SortBufs sb;
 };
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
   return 2 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
-         2 * align256(c * 8) + sortbufs_bytes(c);
+         3 * align256(c * 8) + sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -1404,6 +1453,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, (c / kThreads + 1) * 4, (void **)&g.bsum);
   p = carve(p, c * 8, (void **)&g.masks);
   p = carve(p, c * 8, (void **)&g.rects);
+  p = carve(p, c * 8, (void **)&g.rects2);
   carve_sortbufs(p, c, g.sb);
 }
 
@@ -1493,8 +1543,12 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
                                                 g.bsum);
   // compact visible splats with their depth bits as keys -> Nv in scalars[0];
   // the same launch sums the count blocks' totals -> M in m_out
-  run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA}, nullptr, (uint32_t)n,
-           g.sb.partials, g.scalars, nullptr, s, g.bsum, ncb, (uint32_t *)m_out);
+  const bool exact = gr.conic == nullptr;
+  run_scan(LoadFlag{g.cnt},
+           StoreCompact{depth, g.sb.keyA, g.sb.valA, exact ? g.rects : nullptr,
+                        reinterpret_cast<uint2 *>(g.masks)},
+           nullptr, (uint32_t)n, g.sb.partials, g.scalars, nullptr, s, g.bsum, ncb,
+           (uint32_t *)m_out);
   {
     const int rc = copy_count_to_host(m_out, m_host, event, s);
     if (rc != GSR_OK) return rc;
@@ -1503,16 +1557,23 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // writes each splat's instance count, in depth order, where the keys would go
   uint32_t *kc = g.sb.keyA;
   int32_t *vc = g.sb.valA;
-  // tight binning also needs each splat's depth rank (its emission walks splats
-  // in index order); the exact count kernel's packed rectangles are not used
-  // there, so their array holds it
-  // exact binning: the packed rectangles are also put in depth order (into
-  // the masks array, which only tight binning uses) for the generated first
-  // tile pass of sort_emit
-  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, false, g.cnt,
-            gr.conic != nullptr ? reinterpret_cast<uint32_t *>(g.rects) : nullptr,
-            gr.conic == nullptr ? g.rects : nullptr,
-            gr.conic == nullptr ? reinterpret_cast<uint2 *>(g.masks) : nullptr);
+  RadixOpts o;
+  o.final_keys = false;
+  if (exact) {
+    // the packed rectangles travel with the splats (payload, from the masks
+    // array -- only tight binning uses it otherwise -- and back after the four
+    // passes), so the generated first tile pass of sort_emit reads them in
+    // depth order; the counts are their areas
+    o.pay_cur = reinterpret_cast<uint2 *>(g.masks);
+    o.pay_alt = g.rects2;
+    o.final_area = true;
+  } else {
+    // tight binning: counts gathered by id, and each splat's depth rank (its
+    // emission walks splats in index order), kept in the unused rectangle array
+    o.final_gather = g.cnt;
+    o.final_rank = reinterpret_cast<uint32_t *>(g.rects);
+  }
+  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, o);
   // offsets of each splat's instances, in depth order -> M in scalars[1]
   run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
            s);
@@ -1588,11 +1649,14 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems{st, g.scalars, (uint32_t)n, gr, first},
         (passes == 1 && by_counts) ? nullptr : ib.keyA, v_pass0, g.scalars + 1, (uint32_t)m, 0,
-        bits0, ib.hist, nb, totals1, nullptr, nullptr, nullptr, nullptr);
+        bits0, ib.hist, nb, totals1, ScatterExtras{});
     uint32_t *kc = ib.keyA;
     int32_t *vc = v_pass0;
-    run_radix(kc, vc, ib.keyB, v_pass1, g.scalars + 1, (uint32_t)m, nbits, ib, s, by_keys,
-              nullptr, nullptr, nullptr, nullptr, 1, tc);
+    RadixOpts o;
+    o.final_keys = by_keys;
+    o.p0 = 1;
+    o.final_tc = tc;
+    run_radix(kc, vc, ib.keyB, v_pass1, g.scalars + 1, (uint32_t)m, nbits, ib, s, o);
     ranges(kc);
     return gsr_launch_status("sort_emit");
   }
@@ -1612,8 +1676,11 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
         (uint32_t)n, gr, (uint32_t)m, ib.keyA, v_first);
   uint32_t *kc = ib.keyA;
   int32_t *vc = v_first;
-  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s, by_keys,
-            nullptr, nullptr, nullptr, nullptr, 0, tc, totals1);
+  RadixOpts o;
+  o.final_keys = by_keys;
+  o.final_tc = tc;
+  o.totals0 = totals1;
+  run_radix(kc, vc, ib.keyB, v_second, g.scalars + 1, (uint32_t)m, nbits, ib, s, o);
   ranges(kc);
   return gsr_launch_status("sort_emit");
 }
