@@ -394,7 +394,7 @@ __global__ __launch_bounds__(TW *TH) void mv_blend_fwd_kernel(
 
 // Two pixels per lane (TW*TH/2 threads), as the one-view 16x16 backward; the
 // batch depth is capped at the thread count.
-template <int C, int TW, int TH, int BD>
+template <int C, int TW, int TH, int BD, bool BG>
 __global__ __launch_bounds__(TW *TH / 2) void mv_blend_bwd_kernel(
     const float *__restrict__ uv, const float *__restrict__ conic,
     const float *__restrict__ opacity, const float *__restrict__ feat,
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(TW *TH / 2) void mv_blend_bwd_kernel(
     const int tile = s_tile;
     if (tile >= T_total) break;  // uniform
     const TileView tv = mv_tile_view(s_vi, V, tile, npv, P_total);
-    blend_bwd_tile2<C, TW, TH, BDX, 2>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic,
+    blend_bwd_tile2<C, TW, TH, BDX, 2, BG>(tv, ranges[2 * tile], ranges[2 * tile + 1], uv, conic,
                                        opacity, feat, ids, bg, Tfin, ncon, dout, dTfin, d_splat,
                                        ld, sm, nullptr, nullptr);
   }
@@ -474,12 +474,23 @@ struct LaunchPreBwd {
 };
 
 // blend dispatch over channel count (3, 4, 6, 7), tile shape and batch depth
+// bg_term: the backward keeps the background term (bg != 0 or a T_final gradient); the
+// variant without it exists for 16x16 tiles only
 template <bool FWD, class... A>
-int dispatch_blend(int C, int tw, int th, int bd, unsigned nb, hipStream_t st, A... a) {
+int dispatch_blend(int C, int tw, int th, int bd, bool bg_term, unsigned nb, hipStream_t st,
+                   A... a) {
 #define GSR_K(C_, TW_, TH_, BD_)                                                        \
   if (C == C_ && tw == TW_ && th == TH_ && bd == BD_) {                                 \
-    if constexpr (FWD) mv_blend_fwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_, 0, st>>>(a...); \
-    else mv_blend_bwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_ / 2, 0, st>>>(a...);    \
+    if constexpr (FWD) {                                                                \
+      mv_blend_fwd_kernel<C_, TW_, TH_, BD_><<<nb, TW_ * TH_, 0, st>>>(a...);           \
+    } else if constexpr (TW_ == 16 && TH_ == 16) {                                      \
+      if (bg_term)                                                                      \
+        mv_blend_bwd_kernel<C_, TW_, TH_, BD_, true><<<nb, TW_ * TH_ / 2, 0, st>>>(a...); \
+      else                                                                              \
+        mv_blend_bwd_kernel<C_, TW_, TH_, BD_, false><<<nb, TW_ * TH_ / 2, 0, st>>>(a...); \
+    } else {                                                                            \
+      mv_blend_bwd_kernel<C_, TW_, TH_, BD_, true><<<nb, TW_ * TH_ / 2, 0, st>>>(a...); \
+    }                                                                                   \
     return GSR_OK;                                                                      \
   }
 #define GSR_SHAPES(C_)                                                                  \
@@ -546,7 +557,7 @@ int gsr_mv_alpha_blending_fwd(const float *uv, const float *conic, const float *
   hipStream_t st = (hipStream_t)stream;
   zero_int_kernel<<<1, 1, 0, st>>>(queue);
   const unsigned nb = (unsigned)min(total_tiles, persistent_blocks());
-  const int rc = dispatch_blend<true>(C, tile_w, tile_h, batch_depth, nb, st, uv, conic, opacity,
+  const int rc = dispatch_blend<true>(C, tile_w, tile_h, batch_depth, true, nb, st, uv, conic, opacity,
                                       feature, ids, tile_range, bg, V, view_i, n_per_view,
                                       total_tiles, total_pixels, queue, out, T_final, n_contrib);
   GSR_REQUIRE(rc == GSR_OK, "mv_alpha_blending_fwd: unsupported C=%d / tile %dx%d / depth %d", C,
@@ -568,7 +579,8 @@ int gsr_mv_alpha_blending_bwd(const float *uv, const float *conic, const float *
   zero_int_kernel<<<1, 1, 0, st>>>(queue);
   // half-size workgroups: twice as many keep the same waves per CU
   const unsigned nb = (unsigned)min(total_tiles, 2 * persistent_blocks());
-  const int rc = dispatch_blend<false>(C, tile_w, tile_h, batch_depth, nb, st, uv, conic,
+  const int rc = dispatch_blend<false>(C, tile_w, tile_h, batch_depth,
+                                       bg != 0.0f || d_T_final != nullptr, nb, st, uv, conic,
                                        opacity, feature, ids, tile_range, bg, V, view_i,
                                        n_per_view, total_tiles, total_pixels, queue, T_final,
                                        n_contrib, d_out, d_T_final, d_splat, ld);
