@@ -144,14 +144,18 @@ int gsr_sort_emit(const float *uv, const int32_t *radius, int64_t n, int W, int 
  * inside a captured graph) is safe: if the device count exceeds it the
  * emission is truncated in bounds and the caller detects it from *m_out. */
 
-/* Exact binning generates the items of its first tile radix pass from the
- * depth-ordered splats where the pass uses them, instead of writing the
- * instance list and reading it back; the remaining tile passes and the tile
- * ranges follow as before.  gsr_set_fused_emission(0) makes it write the
- * list first (process-wide; for comparisons), 1 restores the default;
- * returns the previous setting.  Both give the same ids_out and tile_range.
- * No msplat counterpart. */
-int gsr_set_fused_emission(int enable);
+/* Exact binning of one view has three implementations, all giving the same
+ * ids_out and tile_range (process-wide switch, for comparisons; returns the
+ * previous mode, ignores values outside 0..2; no msplat counterpart):
+ *   2 (default): grids of at most 256 x 256 tiles are binned by rows -- one
+ *     segment per splat and rectangle row sorted by the row, then each row's
+ *     segments expanded and sorted by the column, the last pass writing
+ *     ids_out and tile_range; larger grids as 1;
+ *   1: the first tile radix pass generates its items from the depth-ordered
+ *     splats where it uses them, the other passes sort lists;
+ *   0: the instance list is written, then sorted by list passes.
+ * gsr_sort_count prepares mode 2 (one more scan) for every grid it can take. */
+int gsr_set_binning_mode(int mode);
 
 /* gsr_sort_count + gsr_sort_emit in one call that allocates nothing: no
  * counterpart in msplat or in the reference; it is for callers that cannot

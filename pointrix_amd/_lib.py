@@ -42,7 +42,7 @@ SIGNATURES = {
     "gsr_sort_ws_inst_bytes": (_SZ, [_I64]),
     "gsr_sort_count": (_I, [_P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),
     "gsr_sort_emit": (_I, [_P, _P, _I64, _I, _I, _I, _I, _P, _I64, _P, _P, _P, _P]),
-    "gsr_set_fused_emission": (_I, [_I]),
+    "gsr_set_binning_mode": (_I, [_I]),
     "gsr_sort_gaussian_noalloc": (_I, [_P, _P, _P, _I64, _I, _I, _I, _I, _I64, _P, _P, _P, _P]),
     "gsr_mv_preprocess_fwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _I, _I, _F, _F,
                                    _I, _P, _P, _P, _P, _P, _P, _P]),

@@ -429,11 +429,12 @@ __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restri
 // output run instead of scattering single words.
 // Items of a radix block from arrays in memory.
 struct ListItems {
+  static constexpr bool kPackedItems = false;
   const uint32_t *kin;
   const int32_t *vin;
   // rows wbase + 64 r + lane of this wave (every load issued before the
   // ranking, whose wave barriers would otherwise expose one latency per row)
-  __device__ __forceinline__ void load(uint32_t, uint32_t wbase, uint32_t n,
+  __device__ __forceinline__ void load(uint32_t, uint32_t, uint32_t wbase, uint32_t n,
                                        uint32_t (&key)[kItems], int32_t (&val)[kItems],
                                        uint32_t *, int32_t *, uint32_t *) const {
     const int lane = lane_id();
@@ -453,12 +454,21 @@ struct ListItems {
 // position (the inverse permutation); pin / pout: an 8-byte payload per item
 // (pin in the input order) moved with it; kout_area: the key output receives
 // the payload's packed rectangle area (w * h of (x0 | y0 << 16, w | h << 16)).
+// binfo (row binning's pass B): block b covers items [binfo[b].x, binfo[b].y)
+// of row ty = binfo[b].w & 255, whose blocks are blk0 = binfo[b].z onwards
+// (binfo[b].w >> 8 of them); digits are the row's columns, each placed from
+// its tile's start (the row's first instance plus the earlier columns'
+// counts, both from the scanned histogram), and the row's first block writes
+// tile_range for the row's gx tiles.
 struct ScatterExtras {
   const uint32_t *kgather = nullptr;
   uint32_t *rank_out = nullptr;
   const uint2 *pin = nullptr;
   uint2 *pout = nullptr;
   bool kout_area = false;
+  const uint4 *binfo = nullptr;
+  int32_t *tile_range = nullptr;
+  int gx = 0;
 };
 
 // kPay: the pass carries a payload (ex.pin / ex.pout); its LDS index array
@@ -471,12 +481,23 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   static_assert(kRadix == kThreads, "one digit per thread");
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
+  // packed items (Items::kPackedItems): the key word carries the value in its
+  // low 24 bits, and no value array is staged
+  constexpr bool kPk = Items::kPackedItems;
   __shared__ uint32_t s_key[kTileItems];
-  __shared__ int32_t s_val[kTileItems];
+  __shared__ int32_t s_val[kPk ? 1 : kTileItems];
   __shared__ uint16_t s_src[kPay ? kTileItems : 1];  // block-local input index
   __shared__ uint32_t s_flag[kThreads];
-  const uint32_t n = count_of(n_ptr, cap);
-  const uint32_t base = blockIdx.x * kTileItems;
+  uint32_t n, base;
+  uint4 bi = make_uint4(0u, 0u, 0u, 0u);
+  if (ex.binfo != nullptr) {
+    bi = ex.binfo[blockIdx.x];
+    base = bi.x;
+    n = bi.y;
+  } else {
+    n = count_of(n_ptr, cap);
+    base = blockIdx.x * kTileItems;
+  }
   if (base >= n) return;  // uniform over the block
   const uint32_t mask = (1u << bits) - 1u;
   const int w = threadIdx.x >> 6, lane = lane_id();
@@ -488,9 +509,16 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   uint32_t rank[kItems];
   const uint32_t wbase = base + w * (kTileItems / (kThreads / 64));
   // this thread's digit total and block offset, requested with the items
-  const uint32_t my_total = (int)threadIdx.x < (1 << bits) ? totals[threadIdx.x] : 0u;
+  const bool my_digit = (int)threadIdx.x < (1 << bits);
+  const uint32_t my_total = my_digit ? totals[threadIdx.x] : 0u;
   const uint32_t my_hist = hist_scanned[threadIdx.x * nb + blockIdx.x];
-  items.load(blockIdx.x, wbase, n, key, val, s_key, s_val, s_flag);
+  // row blocks: this column's count before the row's first block and after its last
+  uint32_t row_p0 = 0, row_p1 = 0;
+  if (ex.binfo != nullptr && my_digit) {
+    row_p0 = hist_scanned[threadIdx.x * nb + bi.z];
+    row_p1 = hist_scanned[threadIdx.x * nb + bi.z + (bi.w >> 8)];
+  }
+  items.load(blockIdx.x, base, wbase, n, key, val, s_key, s_val, s_flag);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
@@ -526,11 +554,26 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     }
     uint32_t total;
     const uint32_t lstart = block_exclusive_scan(run, total);
-    // global start of digit d: all smaller digits, then d in earlier blocks
-    const uint32_t dstart = block_exclusive_scan(my_total, total);
 #pragma unroll
     for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][d] += lstart;
-    s_delta[d] = (int32_t)(dstart + my_hist - lstart);
+    if (ex.binfo == nullptr) {
+      // global start of digit d: all smaller digits, then d in earlier blocks
+      const uint32_t dstart = block_exclusive_scan(my_total, total);
+      s_delta[d] = (int32_t)(dstart + my_hist - lstart);
+    } else {
+      // tile (ty, d): the row's first instance, the row's smaller columns,
+      // then d in the row's earlier blocks
+      const uint32_t blk0 = bi.z, ty = bi.w & 0xFFu;
+      const uint32_t c = row_p1 - row_p0;
+      const uint32_t e0 = base - (blockIdx.x - blk0) * kTileItems;
+      const uint32_t tstart = e0 + block_exclusive_scan(c, total);
+      s_delta[d] = (int32_t)(tstart + (my_hist - row_p0) - lstart);
+      if (blockIdx.x == blk0 && d < ex.gx) {
+        const int64_t t = (int64_t)ty * ex.gx + d;
+        ex.tile_range[2 * t] = c ? (int32_t)tstart : 0;
+        ex.tile_range[2 * t + 1] = c ? (int32_t)(tstart + c) : 0;
+      }
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -539,7 +582,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     if (i < n) {
       const uint32_t lp = s_cnt[w][(key[r] >> shift) & mask] + rank[r];
       s_key[lp] = key[r];
-      s_val[lp] = val[r];
+      if constexpr (!kPk) s_val[lp] = val[r];
       if constexpr (kPay) s_src[lp] = (uint16_t)(i - base);
     }
   }
@@ -551,7 +594,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     if (i < cnt) {
       const uint32_t kk = s_key[i];
       const uint32_t pos = i + (uint32_t)s_delta[(kk >> shift) & mask];
-      const int32_t vv = s_val[i];
+      const int32_t vv = kPk ? (int32_t)(kk & 0xFFFFFFu) : s_val[i];
       uint2 pl = make_uint2(0u, 0u);
       if constexpr (kPay) {
         // the payload row of this item, read from the block's input rows
@@ -983,6 +1026,7 @@ __global__ __launch_bounds__(kThreads) void emit_rest_kernel(
 // removes the emitted list's write and its re-read by the first pass
 // (8 + 12 bytes per instance).
 struct SplatStream {
+  static constexpr bool kOneRow = false;
   const int32_t *ids;   // depth-sorted splat ids
   const uint32_t *offs; // instance offset of each (strictly increasing)
   const uint2 *rects;   // packed rectangle of each, in depth order
@@ -1015,6 +1059,86 @@ __device__ __forceinline__ EmitWave stream_load(const SplatStream &st, uint32_t 
   return e;
 }
 
+// ---- row binning: segments (one per splat and rectangle row) ---------------
+// A grid of at most 256 x 256 tiles is binned in two generated passes instead
+// of two passes over the instances.  Pass A sorts the segments -- one per
+// rectangle row of each depth-ordered splat, key ty | x0 << 8 | w << 16, value
+// the splat id -- stably by the row ty; pass B expands every segment into its
+// w tiles and sorts them stably by the column tx, each radix block inside one
+// row, so the row-major tile order and the depth order inside a tile follow.
+// Pass A moves S ~ M / 4 items, and pass B writes the final ids and the tile
+// ranges: no pass reads or writes the instance keys.
+//
+// Both passes reuse the generator of the first tile pass: an element with n
+// items is a one-row rectangle of width n (inv_w = 1 / n, row 0), and the
+// generated "tile" of item k is toff + x0 + k (grid width 0).
+//
+// Pass A's elements are the depth-ordered splats: item k = rectangle row
+// y0 + k, key toff + y0 + k with toff = x0 << 8 | w << 16 (the row keeps the
+// low 8 bits).
+struct RowStream {
+  static constexpr bool kOneRow = true;
+  const int32_t *ids;    // depth-sorted splat ids
+  const uint32_t *offs;  // segment offset of each (strictly increasing)
+  const uint2 *rects;    // packed rectangle of each, in depth order
+};
+
+__device__ __forceinline__ EmitWave stream_load(const RowStream &st, uint32_t nv, const Grid &,
+                                                uint32_t sb) {
+  EmitWave e{0, 0, 0u, 1u, 0xFFFFFFFFu, 0u, 1.0f, 0u, 0u};
+  if (sb >= nv) return e;  // uniform per wave
+  const uint32_t s = sb + (uint32_t)lane_id();
+  if (s < nv) {
+    e.id = st.ids[s];
+    const uint2 r = st.rects[s];
+    e.off = st.offs[s];
+    const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16;
+    const uint32_t w = r.y & 0xFFFFu, h = r.y >> 16;
+    const uint32_t hh = h > 1u ? h : 1u;
+    e.toff = (int)((x0 << 8) | (w << 16));
+    e.xy = y0;
+    e.wgx = hh;  // grid width 0
+    e.inv_w = 1.0f / (float)hh;
+    e.end = e.off + h;
+  }
+  const int nsplat = min(64, (int)(nv - sb));
+  e.r0 = (uint32_t)__builtin_amdgcn_readlane((int)e.off, 0);
+  e.r1 = (uint32_t)__builtin_amdgcn_readlane((int)e.end, nsplat - 1);
+  return e;
+}
+
+// Pass B's elements are pass A's output in order: segment s (key, splat id)
+// starting at instance E[s] (the exclusive prefix of the widths); item k =
+// column x0 + k.
+struct SegStream {
+  static constexpr bool kOneRow = true;
+  const int32_t *ids;     // splat id of each segment
+  const uint32_t *offs;   // E: first instance of each (strictly increasing)
+  const uint32_t *keys;   // ty | x0 << 8 | w << 16
+};
+
+__device__ __forceinline__ EmitWave stream_load(const SegStream &st, uint32_t nv, const Grid &,
+                                                uint32_t sb) {
+  EmitWave e{0, 0, 0u, 1u, 0xFFFFFFFFu, 0u, 1.0f, 0u, 0u};
+  if (sb >= nv) return e;  // uniform per wave
+  const uint32_t s = sb + (uint32_t)lane_id();
+  if (s < nv) {
+    e.id = st.ids[s];
+    const uint32_t k = st.keys[s];
+    e.off = st.offs[s];
+    const uint32_t w = k >> 16;
+    const uint32_t ww = w > 1u ? w : 1u;
+    e.xy = (k >> 8) & 0xFFu;
+    e.wgx = ww;  // grid width 0
+    e.inv_w = 1.0f / (float)ww;
+    e.end = e.off + w;
+  }
+  const int nsplat = min(64, (int)(nv - sb));
+  e.r0 = (uint32_t)__builtin_amdgcn_readlane((int)e.off, 0);
+  e.r1 = (uint32_t)__builtin_amdgcn_readlane((int)e.end, nsplat - 1);
+  return e;
+}
+
 // Depth-order index of the splat holding instance q: the last s < nv with
 // offs[s] <= q (offsets strictly increase: listed splats own >= 1 instance);
 // a 64-way search, one probe per lane per step.
@@ -1035,8 +1159,47 @@ __device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ offs
 // Instances [q0, q1) in order, from the splat s0 that holds q0, over
 // consecutive groups of 64 splats; the next group is requested before the
 // current one is walked.
+// gen_rounds for streams of one-row elements (RowStream, SegStream): the key
+// of item q of element j is tb_j + q with tb_j = toff + x0 - off, so a round
+// shuffles two values from each item's owner (tb, id) and computes no row.
 template <class F>
-__device__ __forceinline__ void gen_range(const SplatStream &st, uint32_t nv, const Grid &g,
+__device__ __forceinline__ void gen_rounds1(const EmitWave &e, uint32_t cb, uint32_t ce,
+                                            uint32_t *flag, F &f) {
+  const int lane = lane_id();
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const uint32_t tb = (uint32_t)e.toff + (e.xy & 0xFFFFu) - e.off;
+  for (uint32_t c0 = cb; c0 < ce; c0 += 64) {
+    const int j0 = 63 - __clzll(__ballot(e.off <= c0));  // element holding item c0
+    const uint32_t jend = min((uint32_t)__builtin_amdgcn_readlane((int)e.end, j0), ce);
+    if (jend >= c0 + 64u) {
+      // whole rounds inside element j0 (uniform): no owner search
+      const uint32_t jtb = (uint32_t)__builtin_amdgcn_readlane((int)tb, j0);
+      const int jid = __builtin_amdgcn_readlane(e.id, j0);
+      for (; c0 + 64u <= jend; c0 += 64) {
+        const uint32_t q = c0 + (uint32_t)lane;
+        f(q, jtb + q, jid, true, true);
+      }
+      c0 -= 64;  // the loop increment moves to the first round not yet written
+      continue;
+    }
+    flag[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t p = e.off - c0;  // round position of this element's first item
+    if (p >= 1u && p < 64u) flag[p] = 1u;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t starts = __ballot(flag[lane] != 0u);
+    __builtin_amdgcn_wave_barrier();
+    const int o = j0 + (int)__popcll(starts & upto);
+    const uint32_t otb = (uint32_t)shfl_i((int)tb, o);
+    const int oid = shfl_i(e.id, o);
+    const uint32_t q = c0 + (uint32_t)lane;
+    const bool valid = q < ce;
+    f(q, valid ? otb + q : 0u, oid, valid, false);
+  }
+}
+
+template <class Stream, class F>
+__device__ __forceinline__ void gen_range(const Stream &st, uint32_t nv, const Grid &g,
                                           uint32_t s0, uint32_t q0, uint32_t q1, uint32_t *flag,
                                           F &f) {
   EmitWave cur = stream_load(st, nv, g, s0);
@@ -1044,21 +1207,36 @@ __device__ __forceinline__ void gen_range(const SplatStream &st, uint32_t nv, co
   for (uint32_t sb = s0; c0 < q1; sb += 64) {
     const EmitWave nxt = stream_load(st, nv, g, sb + 64);
     const uint32_t ce = min(q1, cur.r1);
-    gen_rounds(cur, c0, ce, flag, f);
+    if constexpr (Stream::kOneRow)
+      gen_rounds1(cur, c0, ce, flag, f);
+    else
+      gen_rounds(cur, c0, ce, flag, f);
     c0 = ce;
     cur = nxt;
   }
 }
 
+// kFix: item qfix gets the width wfix in its key (pass A: the segment that an
+// instance capacity below the count cuts short).  kPacked: one word per item,
+// id | tile << 24 (pass B: tile = the column, ids below 2^24), no values.
+template <bool kFix = false, bool kPacked = false>
 struct GenStage {
   uint32_t *key;
   int32_t *val;
   uint32_t base;
+  uint32_t qfix, wfix;
   __device__ __forceinline__ void operator()(uint32_t q, uint32_t tile, int id, bool valid,
                                              bool) const {
     if (valid) {
-      key[q - base] = tile;
-      val[q - base] = id;
+      if constexpr (kFix) {
+        if (q == qfix) tile = (tile & 0xFFFFu) | (wfix << 16);
+      }
+      if constexpr (kPacked) {
+        key[q - base] = (uint32_t)id | (tile << 24);
+      } else {
+        key[q - base] = tile;
+        val[q - base] = id;
+      }
     }
   }
 };
@@ -1073,27 +1251,52 @@ struct GenStage {
 // be partial), the full cycles to one counter and the run to a difference
 // array over the digits (+1 at its start, -1 past its end, split at the wrap);
 // a block scan over the digits then gives the counts.  first[4 b + w] = the
-// depth-order index of the splat holding wave w's first instance.
+// stream index of the element holding wave w's first item.
+// The same kernel counts row binning's passes (Stream = RowStream, SegStream).
+// kRows: the blocks are pass B's, each inside one tile row: block b is the
+// row's (b - blk0)-th run of 4096 instances (rows[ty] = {first instance,
+// end, blk0, block count}, 256 rows); binfo[b] = {first, end, blk0, ty |
+// count << 8} ({0, 0, 0, 0}: no row, an empty block).
+template <class Stream, bool kRows = false>
 __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
-    SplatStream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
+    Stream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
     uint32_t cap_m, Grid g, uint32_t mask, uint32_t *__restrict__ hist, uint32_t nb,
-    uint32_t *__restrict__ first) {
+    uint32_t *__restrict__ first, const uint4 *__restrict__ rows, uint4 *__restrict__ binfo) {
   __shared__ int32_t s_d[kRadix + 1];
   __shared__ uint32_t s_full;
+  __shared__ uint4 s_rng;
   s_d[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     s_d[kRadix] = 0;
     s_full = 0;
+    s_rng = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
-  const uint32_t M = count_of(m_ptr, cap_m), nv = count_of(nv_ptr, cap_n);
+  uint32_t base, end;
+  if constexpr (kRows) {
+    const uint4 r = rows[threadIdx.x];
+    const uint32_t b = blockIdx.x;
+    if (r.w > 0u && b >= r.z && b - r.z < r.w) {  // one row matches at most
+      const uint32_t lo = r.x + (b - r.z) * kTileItems;
+      s_rng = make_uint4(lo, min(lo + kTileItems, r.y), r.z, threadIdx.x | (r.w << 8));
+    }
+    __syncthreads();
+    const uint4 bi = s_rng;
+    if (threadIdx.x == 0) binfo[b] = bi;
+    base = bi.x;
+    end = bi.y;
+  } else {
+    base = blockIdx.x * kTileItems;
+    end = count_of(m_ptr, cap_m);
+  }
+  const uint32_t nv = count_of(nv_ptr, cap_n);
   const int w = threadIdx.x >> 6;
   const uint32_t D = mask + 1u;
   const uint32_t dbits = (uint32_t)__popc(mask);
   constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
-  const uint32_t q0 = blockIdx.x * kTileItems + (uint32_t)w * kWaveItems;
-  if (q0 < M) {  // uniform per wave
-    const uint32_t q1 = min(q0 + kWaveItems, M);
+  const uint32_t q0 = base + (uint32_t)w * kWaveItems;
+  if (q0 < end) {  // uniform per wave
+    const uint32_t q1 = min(q0 + kWaveItems, end);
     const uint32_t s0 = find_owner(st.offs, nv, q0);
     if (lane_id() == 0) first[blockIdx.x * (kThreads / 64) + w] = s0;
     EmitWave cur = stream_load(st, nv, g, s0);
@@ -1141,23 +1344,31 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   hist[threadIdx.x * nb + blockIdx.x] = threadIdx.x < D ? s_full + excl + (uint32_t)v : 0u;
 }
 
-// Items of a radix block generated from the depth-ordered splats (the first
-// tile pass): the wave writes its 1024 instances (tile key, splat id) into
-// its quarter of the block's LDS staging arrays, then reads its rows back.
+// Items of a radix block generated from a stream (the first tile pass: the
+// depth-ordered splats; row binning: RowStream, SegStream): the wave writes
+// its 1024 items (key, id) into its quarter of the block's LDS staging
+// arrays, then reads its rows back.  kFix: fix[0], fix[1] = the item and
+// width GenStage corrects (pass A).
+template <class Stream, bool kFix = false, bool kPacked = false>
 struct GenItems {
-  SplatStream st;
+  static constexpr bool kPackedItems = kPacked;
+  Stream st;
   const uint32_t *nv_ptr;
   uint32_t cap_n;
   Grid g;
   const uint32_t *first;
-  __device__ __forceinline__ void load(uint32_t block, uint32_t wbase, uint32_t n,
+  const uint32_t *fix;
+  __device__ __forceinline__ void load(uint32_t block, uint32_t base, uint32_t wbase, uint32_t n,
                                        uint32_t (&key)[kItems], int32_t (&val)[kItems],
                                        uint32_t *s_key, int32_t *s_val, uint32_t *s_flag) const {
     const int w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t base = block * kTileItems;
     constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
     if (wbase < n) {  // uniform per wave
-      GenStage f{s_key, s_val, base};
+      GenStage<kFix, kPacked> f{s_key, s_val, base, 0xFFFFFFFFu, 0u};
+      if constexpr (kFix) {
+        f.qfix = fix[0];
+        f.wfix = fix[1];
+      }
       gen_range(st, count_of(nv_ptr, cap_n), g, first[block * (kThreads / 64) + w], wbase,
                 min(wbase + kWaveItems, n), s_flag + 64 * w, f);
     }
@@ -1168,7 +1379,7 @@ struct GenItems {
       const uint32_t i = wbase + r * 64 + lane;
       const bool valid = i < n;
       key[r] = valid ? s_key[i - base] : 0u;
-      val[r] = valid ? s_val[i - base] : 0;
+      if constexpr (!kPacked) val[r] = valid ? s_val[i - base] : 0;
     }
   }
 };
@@ -1294,6 +1505,80 @@ __global__ __launch_bounds__(kThreads) void ranges_kernel(const uint32_t *__rest
       if (m == M - 1 || k[2 + j] != t) ranges[2 * t + 1] = (int32_t)(m + 1);
     }
   }
+}
+
+// ---- row binning: the small kernels -----------------------------------------
+// Segment count of each depth-ordered splat (its rectangle's rows), scanned
+// into the segment offsets.
+struct LoadRectH {
+  const uint2 *rects;
+  __device__ uint32_t operator()(uint32_t i) const { return rects[i].y >> 16; }
+};
+// Width of each segment in pass A's output, scanned into E.
+struct LoadSegW {
+  const uint32_t *keys;
+  __device__ uint32_t operator()(uint32_t i) const { return keys[i] >> 16; }
+};
+
+// Segments of the first Mc = min(M, m) instances of the depth-first sequence
+// (an instance capacity m below the count M keeps that prefix): sc[0] = Nv,
+// sc[1] = M, sc[6] = S (all segments); writes sc[2] = the segments kept,
+// sc[3] = the last kept one if the capacity cuts it short (else 0xFFFFFFFF)
+// and sc[4] its width, sc[5] = Mc.  One wave.
+__global__ __launch_bounds__(64) void seg_limit_kernel(const uint32_t *__restrict__ offs,
+                                                       const uint32_t *__restrict__ soffs,
+                                                       const uint2 *__restrict__ rects,
+                                                       uint32_t cap_n, uint32_t m,
+                                                       uint32_t *sc) {
+  const uint32_t nv = min(sc[0], cap_n), M = sc[1];
+  const uint32_t mc = min(M, m);
+  uint32_t seff = sc[6], qfix = 0xFFFFFFFFu, wfix = 0u;
+  if (M > m) {  // uniform
+    seff = 0u;
+    if (mc > 0u && nv > 0u) {
+      const uint32_t q = mc - 1u;  // the last kept instance
+      const uint32_t s = find_owner(offs, nv, q);
+      const uint32_t w = max(rects[s].y & 0xFFFFu, 1u);
+      const uint32_t k = q - offs[s];
+      const uint32_t row = k / w;
+      seff = soffs[s] + row + 1u;
+      qfix = seff - 1u;
+      wfix = k - row * w + 1u;
+    }
+  }
+  if (lane_id() == 0) {
+    sc[2] = seff;
+    sc[3] = qfix;
+    sc[4] = wfix;
+    sc[5] = mc;
+  }
+}
+
+// Row table of pass B from pass A's digit totals (segments per row) and E:
+// rows[ty] = {first instance, end, first block, block count} (4096 instances
+// per block, blocks numbered row after row); the tiles of rows without
+// instances get (0, 0) here (the others from pass B's first block of the row).
+__global__ __launch_bounds__(kThreads) void rows_kernel(const uint32_t *__restrict__ seg_per_row,
+                                                        const uint32_t *__restrict__ E,
+                                                        const uint32_t *__restrict__ sc, int gx,
+                                                        int gy, uint4 *__restrict__ rows,
+                                                        int32_t *__restrict__ tile_range) {
+  const int ty = threadIdx.x;
+  const uint32_t seff = sc[2], mc = sc[5];
+  const uint32_t cnt = ty < gy ? seg_per_row[ty] : 0u;
+  uint32_t total;
+  const uint32_t seg0 = block_exclusive_scan(cnt, total);
+  const uint32_t seg1 = seg0 + cnt;
+  const uint32_t e0 = seg0 < seff ? E[seg0] : mc;
+  const uint32_t e1 = seg1 < seff ? E[seg1] : mc;
+  const uint32_t nbr = (e1 - e0 + kTileItems - 1) / kTileItems;
+  const uint32_t blk0 = block_exclusive_scan(nbr, total);
+  rows[ty] = make_uint4(e0, e1, blk0, nbr);
+  if (ty < gy && nbr == 0u)
+    for (int x = 0; x < gx; ++x) {
+      tile_range[2 * ((int64_t)ty * gx + x)] = 0;
+      tile_range[2 * ((int64_t)ty * gx + x) + 1] = 0;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -1423,14 +1708,34 @@ int tile_bits(int T) {
 // writes no keys and no kernel re-reads them.
 constexpr int kCountTiles = 1 << 16;
 
-// Exact binning generates its first tile pass's items from the depth-ordered
-// splats (gen_hist_kernel, radix_scatter_kernel<GenItems>) unless switched
-// off, when the emission writes the instance list and list passes sort it.
-bool g_fused_emission = true;
+// Exact binning of one view: mode 2 (default) bins grids of at most 256 x 256
+// tiles by rows (two generated passes, the second writing ids and ranges) and
+// larger grids as mode 1; mode 1 generates the first tile pass's items from
+// the depth-ordered splats (gen_hist_kernel, radix_scatter_kernel<GenItems>);
+// mode 0 writes the instance list and list passes sort it.
+int g_binning_mode = 2;
+constexpr int kRowGrid = 256;  // row binning: columns and rows per grid at most
+// pass B's launch: at most ceil(m / 4096) + (rows - 1) blocks hold instances
+// (each row adds at most one partial block), plus one empty block whose zero
+// histogram column closes the scanned histogram
+inline uint32_t row_blocks(uint64_t m) { return nblk(m) + kRowGrid + 1; }
+
+// grids row binning can take (sort_count then always writes the segment
+// offsets, so a mode change between count and emit stays consistent)
+// (pass B packs splat ids below 2^24 with the column into one word)
+bool row_grid(const Grid &gr, int T, int64_t n) {
+  return gr.conic == nullptr && gr.vi == nullptr && T > 1 && gr.gx <= kRowGrid &&
+         gr.gy <= kRowGrid && n < (int64_t(1) << 24);
+}
+bool row_binning(const Grid &gr, int T, int64_t n) {
+  return g_binning_mode == 2 && row_grid(gr, T, n);
+}
 
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
-  uint32_t *cnt, *offs, *scalars;  // scalars[0] = Nv, scalars[1] = M
+  // scalars[0] = Nv, scalars[1] = M; row binning: [2..5] (seg_limit_kernel), [6] = S
+  uint32_t *cnt, *offs, *scalars;
+  uint32_t *soffs;                 // row binning: segment offsets, in depth order
   uint32_t *bsum;                  // instances per count block
   uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
   uint2 *rects;                    // exact binning: packed tile rectangles
@@ -1440,7 +1745,7 @@ struct GaussWs {
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
-  return 2 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
+  return 3 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
          3 * align256(c * 8) + sortbufs_bytes(c);
 }
 
@@ -1450,6 +1755,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 4, (void **)&g.cnt);
   p = carve(p, c * 4, (void **)&g.offs);
   p = carve(p, 64, (void **)&g.scalars);
+  p = carve(p, c * 4, (void **)&g.soffs);
   p = carve(p, (c / kThreads + 1) * 4, (void **)&g.bsum);
   p = carve(p, c * 8, (void **)&g.masks);
   p = carve(p, c * 8, (void **)&g.rects);
@@ -1466,14 +1772,18 @@ size_t gsr_sort_ws_gauss_bytes(int64_t n) { return gauss_ws_bytes((uint64_t)(n >
 size_t gsr_sort_ws_inst_bytes(int64_t m) {
   const uint64_t c = (uint64_t)(m > 0 ? m : 1);
   const uint64_t nb = nblk(c);
+  const uint64_t nbr = row_blocks(c);
   return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8) +
          align256(nb * (kThreads / 64) * 4) + align256(nb * 2 * kRadix * 4) +
-         align256(kRadix * 4) + 2 * align256((uint64_t)kCountTiles * 4);
+         align256(kRadix * 4) + 2 * align256((uint64_t)kCountTiles * 4) +
+         // row binning: pass B's first[], histogram, totals, binfo; the row table
+         align256(nbr * (kThreads / 64) * 4) + align256(nbr * kRadix * 4) +
+         align256(kRadix * 4) + align256(nbr * 16) + align256(kRadix * 16);
 }
 
-int gsr_set_fused_emission(int enable) {
-  const int prev = g_fused_emission;
-  g_fused_emission = enable != 0;
+int gsr_set_binning_mode(int mode) {
+  const int prev = g_binning_mode;
+  if (mode >= 0 && mode <= 2) g_binning_mode = mode;
   return prev;
 }
 
@@ -1577,6 +1887,10 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // offsets of each splat's instances, in depth order -> M in scalars[1]
   run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
            s);
+  // row binning: offsets of each splat's segments (rectangle rows) -> S in scalars[6]
+  if (row_grid(gr, gr.gx * gr.gy, n))
+    run_scan(LoadRectH{reinterpret_cast<const uint2 *>(g.masks)}, StoreU32{g.soffs}, g.scalars,
+             cap, g.sb.partials, g.scalars + 6, nullptr, s);
   return gsr_launch_status("sort_count");
 }
 
@@ -1603,7 +1917,49 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   p = carve(p, (uint64_t)nb * 2 * kRadix * 4, (void **)&tcnt);
   p = carve(p, (uint64_t)kRadix * 4, (void **)&totals1);
   p = carve(p, (uint64_t)kCountTiles * 4, (void **)&over);
-  carve(p, (uint64_t)kCountTiles * 4, (void **)&counts);
+  p = carve(p, (uint64_t)kCountTiles * 4, (void **)&counts);
+  if (row_binning(gr, T, n)) {
+    const uint32_t nbB = row_blocks((uint64_t)m);
+    uint32_t *firstB, *histB, *totalsB;
+    uint4 *binfo, *rows;
+    p = carve(p, (uint64_t)nbB * (kThreads / 64) * 4, (void **)&firstB);
+    p = carve(p, (uint64_t)nbB * kRadix * 4, (void **)&histB);
+    p = carve(p, (uint64_t)kRadix * 4, (void **)&totalsB);
+    p = carve(p, (uint64_t)nbB * 16, (void **)&binfo);
+    carve(p, (uint64_t)kRadix * 16, (void **)&rows);
+    const uint32_t mu = (uint32_t)m;
+    const int bitsA = tile_bits(gr.gy), bitsB = tile_bits(gr.gx);
+    const uint2 *rects = reinterpret_cast<const uint2 *>(g.masks);
+    seg_limit_kernel<<<1, 64, 0, s>>>(g.offs, g.soffs, rects, (uint32_t)n, mu, g.scalars);
+    // pass A: segments of the depth-ordered splats, stably by row
+    const RowStream rs{g.sb.valA, g.soffs, rects};
+    gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, g.scalars + 2,
+                                                       mu, gr, (1u << bitsA) - 1u, ib.hist, nb,
+                                                       first, nullptr, nullptr);
+    digit_scan_kernel<<<1u << bitsA, kThreads, 0, s>>>(ib.hist, nb, totals1);
+    radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
+        GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
+        ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, ib.hist, nb, totals1, ScatterExtras{});
+    // E: each segment's first instance in pass A's order
+    run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, nullptr,
+             nullptr, s);
+    rows_kernel<<<1, kThreads, 0, s>>>(totals1, ib.keyB, g.scalars, gr.gx, gr.gy, rows,
+                                       tile_range);
+    // pass B: each row's instances, stably by column, straight to ids_out
+    const SegStream ss{ib.valA, ib.keyB, ib.keyA};
+    gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
+        ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
+        rows, binfo);
+    digit_scan_kernel<<<1u << bitsB, kThreads, 0, s>>>(histB, nbB, totalsB);
+    ScatterExtras ex;
+    ex.binfo = binfo;
+    ex.tile_range = tile_range;
+    ex.gx = gr.gx;
+    radix_scatter_kernel<<<nbB, kThreads, 0, s>>>(
+        GenItems<SegStream, false, true>{ss, g.scalars + 2, mu, gr, firstB, nullptr}, nullptr,
+        ids_out, g.scalars + 5, mu, 24, bitsB, histB, nbB, totalsB, ex);
+    return gsr_launch_status("sort_emit");
+  }
   const int nbits = tile_bits(T);
   const int passes = radix_passes(nbits);
   // ranges from counts: one pass (its digit totals are the tile counts) or two
@@ -1634,7 +1990,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     }
     ranges_from_counts_kernel<<<1, 1024, 0, s>>>(cnt, T, tile_range);
   };
-  if (gr.conic == nullptr && passes >= 1 && g_fused_emission) {
+  if (gr.conic == nullptr && passes >= 1 && g_binning_mode >= 1) {
     // first tile pass over generated items (sort_count's last depth pass put
     // the rectangles in depth order into the masks array), then the rest as
     // list passes; values alternate so that the last pass writes ids_out
@@ -1642,12 +1998,13 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     int32_t *v_pass1 = (passes % 2 == 1) ? ib.valA : ids_out;
     const int bits0 = min(per, nbits);
     const SplatStream st{g.sb.valA, g.offs, reinterpret_cast<const uint2 *>(g.masks)};
-    gen_hist_kernel<<<nb, kThreads, 0, s>>>(st, g.scalars, (uint32_t)n, g.scalars + 1,
-                                            (uint32_t)m, gr, (1u << bits0) - 1u, ib.hist, nb,
-                                            first);
+    gen_hist_kernel<SplatStream><<<nb, kThreads, 0, s>>>(st, g.scalars, (uint32_t)n,
+                                                         g.scalars + 1, (uint32_t)m, gr,
+                                                         (1u << bits0) - 1u, ib.hist, nb, first,
+                                                         nullptr, nullptr);
     digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, totals1);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        GenItems{st, g.scalars, (uint32_t)n, gr, first},
+        GenItems<SplatStream>{st, g.scalars, (uint32_t)n, gr, first, nullptr},
         (passes == 1 && by_counts) ? nullptr : ib.keyA, v_pass0, g.scalars + 1, (uint32_t)m, 0,
         bits0, ib.hist, nb, totals1, ScatterExtras{});
     uint32_t *kc = ib.keyA;

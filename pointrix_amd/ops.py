@@ -275,12 +275,16 @@ class SortWorkspace:
         return self.m_dev
 
 
-def set_fused_emission(enable: bool) -> bool:
-    """Exact binning generates its first tile pass's items from the
-    depth-ordered splats (the default); ``False`` writes the instance list
-    first and sorts it with list passes.  Both give the same ids and tile
-    ranges.  Process-wide; returns the previous setting."""
-    return bool(_lib.load().gsr_set_fused_emission(int(bool(enable))))
+def set_binning_mode(mode: int) -> int:
+    """Implementation of exact one-view binning, process-wide: 2 (default)
+    bins grids of at most 256 x 256 tiles by rows (segments sorted by row,
+    then expanded and sorted by column); 1 generates the first tile radix
+    pass's items from the depth-ordered splats; 0 writes the instance list
+    and sorts it.  All give the same ids and tile ranges.  Returns the
+    previous mode."""
+    if mode not in (0, 1, 2):
+        raise ValueError(f"binning mode must be 0, 1 or 2, not {mode!r}")
+    return int(_lib.load().gsr_set_binning_mode(int(mode)))
 
 
 _default_ws = SortWorkspace()
