@@ -59,118 +59,160 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &t
 }
 
 // --------------------------------------------------------------------------
-// Generic three-kernel exclusive scan (reduce / scan partials / downsweep).
+// Generic two-kernel exclusive scan (block sums / downsweep) of uint32_t
+// values, or of uint2 values (two independent sums in one pass).
 // --------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t vadd(uint32_t a, uint32_t b) { return a + b; }
+__device__ __forceinline__ uint2 vadd(uint2 a, uint2 b) {
+  return make_uint2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ uint32_t vsub(uint32_t a, uint32_t b) { return a - b; }
+__device__ __forceinline__ uint2 vsub(uint2 a, uint2 b) {
+  return make_uint2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ uint32_t vshfl_up(uint32_t x, int d) { return __shfl_up(x, d, 64); }
+__device__ __forceinline__ uint2 vshfl_up(uint2 x, int d) {
+  return make_uint2(__shfl_up(x.x, d, 64), __shfl_up(x.y, d, 64));
+}
+template <class V>
+__device__ __forceinline__ V vzero() {
+  return V{};
+}
+
+// block_exclusive_scan over V
+template <class V>
+__device__ __forceinline__ V block_exclusive_scan_v(V v, V &total) {
+  __shared__ V s_wv[kThreads / 64];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  V inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const V t = vshfl_up(inc, d);
+    if (lane >= d) inc = vadd(inc, t);
+  }
+  if (lane == 63) s_wv[w] = inc;
+  __syncthreads();
+  V base = vzero<V>();
+  total = vzero<V>();
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) {
+    const V s = s_wv[k];
+    if (k < w) base = vadd(base, s);
+    total = vadd(total, s);
+  }
+  __syncthreads();
+  return vadd(base, vsub(inc, v));
+}
+
 template <class Load>
-__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(Load ld, const uint32_t *n_ptr,
-                                                               uint32_t cap,
-                                                               uint32_t *partials) {
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(
+    Load ld, const uint32_t *n_ptr, uint32_t cap, typename Load::value_type *partials) {
+  using V = typename Load::value_type;
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
-  uint32_t s = 0;
+  V s = vzero<V>();
   if (base < n) {
 #pragma unroll 4
     for (int k = 0; k < kItems; ++k) {
       const uint32_t i = base + k * kThreads + threadIdx.x;
-      if (i < n) s += ld(i);
+      if (i < n) s = vadd(s, ld(i));
     }
   }
-  uint32_t total;
-  block_exclusive_scan(s, total);
+  V total;
+  block_exclusive_scan_v(s, total);
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// One 1024-thread block scans the per-block partials in place (exclusive) and
-// writes the grand total.
-__global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t *partials, uint32_t nb,
-                                                             uint32_t *total_out,
-                                                             uint32_t *total_out2,
-                                                             const uint32_t *__restrict__ aux,
-                                                             uint32_t naux,
-                                                             uint32_t *__restrict__ aux_out) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
-  const int lane = lane_id(), w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < nb; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < nb ? partials[i] : 0u;
-    uint32_t inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += t;
-    }
-    if (lane == 63) s_w[w] = inc;
-    __syncthreads();
-    uint32_t wb = 0, tot = 0;
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t s = s_w[k];
-      if (k < w) wb += s;
-      tot += s;
-    }
-    const uint32_t carry = s_carry;
-    if (i < nb) partials[i] = carry + wb + inc - v;
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry = carry + tot;
-    __syncthreads();
+// What block 0 of a downsweep does with the grand total (all threads of the
+// block call it).
+struct TotNone {
+  template <class V>
+  __device__ void operator()(V) const {}
+};
+struct TotU32 {
+  uint32_t *out;
+  __device__ void operator()(uint32_t t) const {
+    if (threadIdx.x == 0) *out = t;
   }
-  if (threadIdx.x == 0) {
-    if (total_out) *total_out = s_carry;
-    if (total_out2) *total_out2 = s_carry;
-  }
-  if (aux != nullptr) {  // plain sum of aux[0, naux) -> *aux_out
+};
+// the compaction: the total (visible splats) and the plain sum of aux[0, naux)
+// (the count blocks' instance totals: M)
+struct TotCompact {
+  uint32_t *out;
+  const uint32_t *aux;
+  uint32_t naux;
+  uint32_t *aux_out;
+  __device__ void operator()(uint32_t t) const {
     uint32_t a = 0;
 #pragma unroll 4
-    for (uint32_t i = threadIdx.x; i < naux; i += 1024) a += aux[i];
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) a += __shfl_xor(a, d, 64);
-    __syncthreads();
-    if (lane == 0) s_w[w] = a;
-    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < naux; i += kThreads) a += aux[i];
+    uint32_t sum;
+    block_exclusive_scan(a, sum);
     if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (int k = 0; k < 16; ++k) t += s_w[k];
-      *aux_out = t;
+      *out = t;
+      *aux_out = sum;
     }
   }
-}
+};
+// two sums: .x -> out_x, .y -> out_y
+struct TotPair {
+  uint32_t *out_x, *out_y;
+  __device__ void operator()(uint2 t) const {
+    if (threadIdx.x == 0) {
+      *out_x = t.x;
+      *out_y = t.y;
+    }
+  }
+};
 
 // Downsweep: thread t owns items [16t, 16t + 16) of the block for the serial
 // part of the scan, but items are loaded and stored in coalesced order
 // (item r * 256 + t by thread t) through an LDS transpose (row padded to 17
-// words: conflict-free both ways).
-template <class Load, class Store>
-__global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, Store st,
-                                                             const uint32_t *n_ptr, uint32_t cap,
-                                                             const uint32_t *partials) {
+// values: conflict-free both ways).  The block's offset is the sum of the
+// block sums before it (read here: no separate scan of the block sums);
+// block 0 sums them all for the grand total, also when the scan is empty.
+template <class Load, class Store, class Tot>
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(
+    Load ld, Store st, const uint32_t *n_ptr, uint32_t cap,
+    const typename Load::value_type *__restrict__ partials, uint32_t nb, Tot tot) {
+  using V = typename Load::value_type;
   constexpr int kRow = kItems + 1;
-  __shared__ uint32_t s_v[kThreads * kRow];
-  __shared__ uint32_t s_e[kThreads * kRow];
+  __shared__ V s_v[kThreads * kRow];
+  __shared__ V s_e[kThreads * kRow];
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t base = blockIdx.x * kTileItems;
-  if (base >= n) return;
+  if (base >= n && blockIdx.x != 0) return;  // uniform over the block
+  const uint32_t lim = blockIdx.x == 0 ? nb : blockIdx.x;
+  V pre = vzero<V>();
+#pragma unroll 8
+  for (uint32_t k = threadIdx.x; k < lim; k += kThreads) pre = vadd(pre, partials[k]);
+  V before;
+  block_exclusive_scan_v(pre, before);
+  if (blockIdx.x == 0) {
+    tot(before);
+    before = vzero<V>();
+  }
+  if (base >= n) return;  // block 0 of an empty scan
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t idx = r * kThreads + threadIdx.x;
     const uint32_t i = base + idx;
-    s_v[(idx / kItems) * kRow + idx % kItems] = i < n ? ld(i) : 0u;
+    s_v[(idx / kItems) * kRow + idx % kItems] = i < n ? ld(i) : vzero<V>();
   }
   __syncthreads();
-  uint32_t v[kItems];
-  uint32_t s = 0;
+  V v[kItems];
+  V s = vzero<V>();
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     v[k] = s_v[threadIdx.x * kRow + k];
-    s += v[k];
+    s = vadd(s, v[k]);
   }
-  uint32_t total;
-  uint32_t run = partials[blockIdx.x] + block_exclusive_scan(s, total);
+  V total;
+  V run = vadd(before, block_exclusive_scan_v(s, total));
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     s_e[threadIdx.x * kRow + k] = run;
-    run += v[k];
+    run = vadd(run, v[k]);
   }
   __syncthreads();
 #pragma unroll
@@ -183,6 +225,7 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(Load ld, Store st,
 }
 
 struct LoadU32 {
+  using value_type = uint32_t;
   const uint32_t *p;
   __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
 };
@@ -818,6 +861,7 @@ __global__ __launch_bounds__(kThreads) void count_tight_kernel(const float *__re
 }
 
 struct LoadFlag {
+  using value_type = uint32_t;
   const uint32_t *cnt;
   __device__ uint32_t operator()(uint32_t i) const { return cnt[i] > 0 ? 1u : 0u; }
 };
@@ -1508,14 +1552,26 @@ __global__ __launch_bounds__(kThreads) void ranges_kernel(const uint32_t *__rest
 }
 
 // ---- row binning: the small kernels -----------------------------------------
-// Segment count of each depth-ordered splat (its rectangle's rows), scanned
-// into the segment offsets.
-struct LoadRectH {
+// Instance and segment count (rectangle area and rows) of each depth-ordered
+// splat, scanned together into the instance and the segment offsets.
+struct LoadAreaH {
+  using value_type = uint2;
   const uint2 *rects;
-  __device__ uint32_t operator()(uint32_t i) const { return rects[i].y >> 16; }
+  __device__ uint2 operator()(uint32_t i) const {
+    const uint32_t wh = rects[i].y;
+    return make_uint2((wh & 0xFFFFu) * (wh >> 16), wh >> 16);
+  }
+};
+struct StoreOffs2 {
+  uint32_t *offs, *soffs;
+  __device__ void operator()(uint32_t i, uint2 excl, uint2) const {
+    offs[i] = excl.x;
+    soffs[i] = excl.y;
+  }
 };
 // Width of each segment in pass A's output, scanned into E.
 struct LoadSegW {
+  using value_type = uint32_t;
   const uint32_t *keys;
   __device__ uint32_t operator()(uint32_t i) const { return keys[i] >> 16; }
 };
@@ -1621,15 +1677,12 @@ inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b) {
   return p;
 }
 
-template <class Load, class Store>
-void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap, uint32_t *partials,
-              uint32_t *total_out, uint32_t *total_out2, hipStream_t s,
-              const uint32_t *aux = nullptr, uint32_t naux = 0, uint32_t *aux_out = nullptr) {
+template <class Load, class Store, class Tot>
+void run_scan(Load ld, Store st, const uint32_t *n_ptr, uint32_t cap,
+              typename Load::value_type *partials, Tot tot, hipStream_t s) {
   const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
   scan_reduce_kernel<<<nb, kThreads, 0, s>>>(ld, n_ptr, cap, partials);
-  scan_partials_kernel<<<1, 1024, 0, s>>>(partials, nb, total_out, total_out2, aux, naux,
-                                          aux_out);
-  scan_down_kernel<<<nb, kThreads, 0, s>>>(ld, st, n_ptr, cap, partials);
+  scan_down_kernel<<<nb, kThreads, 0, s>>>(ld, st, n_ptr, cap, partials, nb, tot);
 }
 
 inline int radix_passes(int nbits) { return (nbits + kRadixBits - 1) / kRadixBits; }
@@ -1720,15 +1773,13 @@ constexpr int kRowGrid = 256;  // row binning: columns and rows per grid at most
 // histogram column closes the scanned histogram
 inline uint32_t row_blocks(uint64_t m) { return nblk(m) + kRowGrid + 1; }
 
-// grids row binning can take (sort_count then always writes the segment
-// offsets, so a mode change between count and emit stays consistent)
-// (pass B packs splat ids below 2^24 with the column into one word)
-bool row_grid(const Grid &gr, int T, int64_t n) {
-  return gr.conic == nullptr && gr.vi == nullptr && T > 1 && gr.gx <= kRowGrid &&
-         gr.gy <= kRowGrid && n < (int64_t(1) << 24);
-}
+// Row binning's grids: exact, one view, at most 256 x 256 tiles, splat ids
+// below 2^24 (pass B packs them with the column into one word).  sort_count
+// writes the segment offsets for every exact binning, so a mode change
+// between count and emit stays consistent.
 bool row_binning(const Grid &gr, int T, int64_t n) {
-  return g_binning_mode == 2 && row_grid(gr, T, n);
+  return g_binning_mode == 2 && gr.conic == nullptr && gr.vi == nullptr && T > 1 &&
+         gr.gx <= kRowGrid && gr.gy <= kRowGrid && n < (int64_t(1) << 24);
 }
 
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
@@ -1740,13 +1791,14 @@ struct GaussWs {
   uint64_t *masks;                 // tight binning: accepted tiles of rectangles <= 64
   uint2 *rects;                    // exact binning: packed tile rectangles
   uint2 *rects2;                   // their ping-pong partner in the depth sort
+  uint2 *partials2;                // block sums of the two-value offset scan
   SortBufs sb;
 };
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
   return 3 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
-         3 * align256(c * 8) + sortbufs_bytes(c);
+         3 * align256(c * 8) + align256((nblk(c) + 1) * 8) + sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -1760,6 +1812,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 8, (void **)&g.masks);
   p = carve(p, c * 8, (void **)&g.rects);
   p = carve(p, c * 8, (void **)&g.rects2);
+  p = carve(p, (nblk(c) + 1) * 8, (void **)&g.partials2);
   carve_sortbufs(p, c, g.sb);
 }
 
@@ -1857,14 +1910,15 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   run_scan(LoadFlag{g.cnt},
            StoreCompact{depth, g.sb.keyA, g.sb.valA, exact ? g.rects : nullptr,
                         reinterpret_cast<uint2 *>(g.masks)},
-           nullptr, (uint32_t)n, g.sb.partials, g.scalars, nullptr, s, g.bsum, ncb,
-           (uint32_t *)m_out);
+           nullptr, (uint32_t)n, g.sb.partials, TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out},
+           s);
   {
     const int rc = copy_count_to_host(m_out, m_host, event, s);
     if (rc != GSR_OK) return rc;
   }
-  // depth sort of the visible splats (4 passes over 32 bits); the last pass
-  // writes each splat's instance count, in depth order, where the keys would go
+  // depth sort of the visible splats (4 passes over 32 bits); for tight
+  // binning the last pass writes each splat's instance count, in depth order,
+  // where the keys would go
   uint32_t *kc = g.sb.keyA;
   int32_t *vc = g.sb.valA;
   RadixOpts o;
@@ -1872,11 +1926,10 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   if (exact) {
     // the packed rectangles travel with the splats (payload, from the masks
     // array -- only tight binning uses it otherwise -- and back after the four
-    // passes), so the generated first tile pass of sort_emit reads them in
-    // depth order; the counts are their areas
+    // passes), so the offset scan and the generated passes of sort_emit read
+    // them in depth order (the counts are their areas)
     o.pay_cur = reinterpret_cast<uint2 *>(g.masks);
     o.pay_alt = g.rects2;
-    o.final_area = true;
   } else {
     // tight binning: counts gathered by id, and each splat's depth rank (its
     // emission walks splats in index order), kept in the unused rectangle array
@@ -1884,13 +1937,14 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     o.final_rank = reinterpret_cast<uint32_t *>(g.rects);
   }
   run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, o);
-  // offsets of each splat's instances, in depth order -> M in scalars[1]
-  run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials, g.scalars + 1, nullptr,
-           s);
-  // row binning: offsets of each splat's segments (rectangle rows) -> S in scalars[6]
-  if (row_grid(gr, gr.gx * gr.gy, n))
-    run_scan(LoadRectH{reinterpret_cast<const uint2 *>(g.masks)}, StoreU32{g.soffs}, g.scalars,
-             cap, g.sb.partials, g.scalars + 6, nullptr, s);
+  // offsets of each splat's instances, in depth order -> M in scalars[1]; exact
+  // binning also those of its segments (rectangle rows) -> S in scalars[6]
+  if (exact)
+    run_scan(LoadAreaH{reinterpret_cast<const uint2 *>(g.masks)}, StoreOffs2{g.offs, g.soffs},
+             g.scalars, cap, g.partials2, TotPair{g.scalars + 1, g.scalars + 6}, s);
+  else
+    run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials,
+             TotU32{g.scalars + 1}, s);
   return gsr_launch_status("sort_count");
 }
 
@@ -1941,8 +1995,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
         GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
         ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, ib.hist, nb, totals1, ScatterExtras{});
     // E: each segment's first instance in pass A's order
-    run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, nullptr,
-             nullptr, s);
+    run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, TotNone{}, s);
     rows_kernel<<<1, kThreads, 0, s>>>(totals1, ib.keyB, g.scalars, gr.gx, gr.gy, rows,
                                        tile_range);
     // pass B: each row's instances, stably by column, straight to ids_out
