@@ -787,34 +787,53 @@ __device__ __forceinline__ void store_block_sum(uint32_t c, uint32_t *__restrict
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
+// Exact counts, 4096 splats per block (16 per thread, every load issued
+// first): per splat its rectangle's tile count and the rectangle packed as
+// (x0 | y0 << 16, w | h << 16); per block the instance total (bsum, summed
+// into M) and the number of splats with instances (vsum, the block sums of
+// the compaction scan, which then needs no reduction kernel).
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
                                                          const int32_t *__restrict__ radius,
                                                          uint32_t n, Grid g,
                                                          uint32_t *__restrict__ cnt,
                                                          uint2 *__restrict__ rects,
-                                                         uint32_t *__restrict__ bsum) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  uint32_t c = 0;
-  if (i < n) {
-    const int r = radius[i];
-    // the position is requested with the radius (one memory round trip); the
-    // empty asm consumes it here so the compiler cannot sink the load into
-    // the branch
-    const float2 puv = ld_f2(uv + 2 * (int64_t)i);
-    const float pu = puv.x, pv = puv.y;
-    asm volatile("" : : "v"(pu), "v"(pv));
-    uint2 packed = make_uint2(0u, 0u);
-    if (r > 0) {
-      const ItemGrid ig = item_grid(g, i);
-      const Rect q = tile_rect(pu, pv, r, ig.gx, ig.gy, g.tw, g.th);
-      const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
-      c = w * h;
-      packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
-    }
-    cnt[i] = c;
-    rects[i] = packed;
+                                                         uint32_t *__restrict__ bsum,
+                                                         uint32_t *__restrict__ vsum) {
+  const uint32_t base = blockIdx.x * kTileItems;
+  int r[kItems];
+  float2 p[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = base + k * kThreads + threadIdx.x;
+    r[k] = i < n ? radius[i] : 0;
+    p[k] = i < n ? ld_f2(uv + 2 * (int64_t)i) : make_float2(0.f, 0.f);
   }
-  store_block_sum(c, bsum);
+  uint2 sums = make_uint2(0u, 0u);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t i = base + k * kThreads + threadIdx.x;
+    if (i < n) {
+      uint32_t c = 0;
+      uint2 packed = make_uint2(0u, 0u);
+      if (r[k] > 0) {
+        const ItemGrid ig = item_grid(g, i);
+        const Rect q = tile_rect(p[k].x, p[k].y, r[k], ig.gx, ig.gy, g.tw, g.th);
+        const uint32_t w = (uint32_t)(q.x1 - q.x0), h = (uint32_t)(q.y1 - q.y0);
+        c = w * h;
+        packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
+      }
+      cnt[i] = c;
+      rects[i] = packed;
+      sums.x += c;
+      sums.y += c > 0u ? 1u : 0u;
+    }
+  }
+  uint2 tot;
+  block_exclusive_scan_v(sums, tot);
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = tot.x;
+    vsum[blockIdx.x] = tot.y;
+  }
 }
 
 // Tight counts: per splat the number of rectangle tiles tile_reachable
@@ -1071,6 +1090,7 @@ __global__ __launch_bounds__(kThreads) void emit_rest_kernel(
 // (8 + 12 bytes per instance).
 struct SplatStream {
   static constexpr bool kOneRow = false;
+  static constexpr bool kLimit = false;
   const int32_t *ids;   // depth-sorted splat ids
   const uint32_t *offs; // instance offset of each (strictly increasing)
   const uint2 *rects;   // packed rectangle of each, in depth order
@@ -1122,9 +1142,14 @@ __device__ __forceinline__ EmitWave stream_load(const SplatStream &st, uint32_t 
 // low 8 bits).
 struct RowStream {
   static constexpr bool kOneRow = true;
+  static constexpr bool kLimit = true;
   const int32_t *ids;    // depth-sorted splat ids
   const uint32_t *offs;  // segment offset of each (strictly increasing)
   const uint2 *rects;    // packed rectangle of each, in depth order
+  // pass A's histogram computes the segment limit (seg_limit) from these
+  const uint32_t *ioffs;  // instance offset of each splat
+  uint32_t *sc;           // scalars: [0] Nv, [1] M, [6] S in; [2..5] written
+  uint32_t m;             // instance capacity
 };
 
 __device__ __forceinline__ EmitWave stream_load(const RowStream &st, uint32_t nv, const Grid &,
@@ -1156,9 +1181,15 @@ __device__ __forceinline__ EmitWave stream_load(const RowStream &st, uint32_t nv
 // column x0 + k.
 struct SegStream {
   static constexpr bool kOneRow = true;
+  static constexpr bool kLimit = false;
   const int32_t *ids;     // splat id of each segment
   const uint32_t *offs;   // E: first instance of each (strictly increasing)
   const uint32_t *keys;   // ty | x0 << 8 | w << 16
+  // pass B's histogram builds the row table (row_table) from these
+  const uint32_t *seg_per_row;  // pass A's digit totals
+  const uint32_t *sc;           // scalars: [2] segments kept, [5] instances kept
+  int gx, gy;
+  int32_t *tile_range;          // the tiles of rows without instances get (0, 0)
 };
 
 __device__ __forceinline__ EmitWave stream_load(const SegStream &st, uint32_t nv, const Grid &,
@@ -1285,6 +1316,52 @@ struct GenStage {
   }
 };
 
+// Segments of the first Mc = min(M, m) instances of the depth-first sequence
+// (an instance capacity m below the count M keeps that prefix), from sc[0] =
+// Nv, sc[1] = M, sc[6] = S (all segments): {segments kept, the last kept one
+// if the capacity cuts it short (else 0xFFFFFFFF), its width, Mc}.  Called by
+// a whole wave (the search is wave-wide).
+__device__ __forceinline__ uint4 seg_limit(const uint32_t *__restrict__ offs,
+                                           const uint32_t *__restrict__ soffs,
+                                           const uint2 *__restrict__ rects, uint32_t cap_n,
+                                           uint32_t m, const uint32_t *sc) {
+  const uint32_t nv = min(sc[0], cap_n), M = sc[1];
+  const uint32_t mc = min(M, m);
+  uint4 r = make_uint4(sc[6], 0xFFFFFFFFu, 0u, mc);
+  if (M > m) {  // uniform
+    r.x = 0u;
+    if (mc > 0u && nv > 0u) {
+      const uint32_t q = mc - 1u;  // the last kept instance
+      const uint32_t s = find_owner(offs, nv, q);
+      const uint32_t w = max(rects[s].y & 0xFFFFu, 1u);
+      const uint32_t k = q - offs[s];
+      const uint32_t row = k / w;
+      r.x = soffs[s] + row + 1u;
+      r.y = r.x - 1u;
+      r.z = k - row * w + 1u;
+    }
+  }
+  return r;
+}
+
+// Pass B's row table, one row per thread of the block: row ty's segments are
+// [seg0, seg1) of pass A's output (prefix of the digit totals), its instances
+// [E[seg0], E[seg1]) (Mc past the last segment), cut into ceil(count / 4096)
+// blocks numbered row after row from blk0.  -> {e0, e1, blk0, block count}
+__device__ __forceinline__ uint4 row_table(const SegStream &st) {
+  const int ty = threadIdx.x;
+  const uint32_t seff = st.sc[2], mc = st.sc[5];
+  const uint32_t cnt = ty < st.gy ? st.seg_per_row[ty] : 0u;
+  uint32_t total;
+  const uint32_t seg0 = block_exclusive_scan(cnt, total);
+  const uint32_t seg1 = seg0 + cnt;
+  const uint32_t e0 = seg0 < seff ? st.offs[seg0] : mc;
+  const uint32_t e1 = seg1 < seff ? st.offs[seg1] : mc;
+  const uint32_t nbr = (e1 - e0 + kTileItems - 1) / kTileItems;
+  const uint32_t blk0 = block_exclusive_scan(nbr, total);
+  return make_uint4(e0, e1, blk0, nbr);
+}
+
 // Histogram of the first tile pass (digit = tile & mask, D = mask + 1 a power
 // of two <= 256) per radix block, as radix_hist_kernel would count the emitted
 // list, without generating the instances: a splat's instances in one row of
@@ -1297,15 +1374,19 @@ struct GenStage {
 // a block scan over the digits then gives the counts.  first[4 b + w] = the
 // stream index of the element holding wave w's first item.
 // The same kernel counts row binning's passes (Stream = RowStream, SegStream).
+// Pass A (Stream::kLimit): the items are the segments kept (seg_limit, which
+// every block computes -- two reads unless the capacity cuts -- and block 0
+// stores in sc[2..5] for the later kernels).
 // kRows: the blocks are pass B's, each inside one tile row: block b is the
-// row's (b - blk0)-th run of 4096 instances (rows[ty] = {first instance,
-// end, blk0, block count}, 256 rows); binfo[b] = {first, end, blk0, ty |
-// count << 8} ({0, 0, 0, 0}: no row, an empty block).
+// row's (b - blk0)-th run of 4096 instances (row_table, which every block
+// builds); binfo[b] = {first, end, blk0, ty | count << 8} ({0, 0, 0, 0}: no
+// row, an empty block).  The last block, never a row's, clears the ranges of
+// the rows without instances.
 template <class Stream, bool kRows = false>
 __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     Stream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
     uint32_t cap_m, Grid g, uint32_t mask, uint32_t *__restrict__ hist, uint32_t nb,
-    uint32_t *__restrict__ first, const uint4 *__restrict__ rows, uint4 *__restrict__ binfo) {
+    uint32_t *__restrict__ first, uint4 *__restrict__ binfo) {
   __shared__ int32_t s_d[kRadix + 1];
   __shared__ uint32_t s_full;
   __shared__ uint4 s_rng;
@@ -1317,13 +1398,35 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   }
   __syncthreads();
   uint32_t base, end;
-  if constexpr (kRows) {
-    const uint4 r = rows[threadIdx.x];
+  if constexpr (Stream::kLimit) {
+    if (threadIdx.x < 64) {  // wave 0
+      const uint4 lim = seg_limit(st.ioffs, st.offs, st.rects, cap_n, st.m, st.sc);
+      if (threadIdx.x == 0) {
+        s_rng = lim;
+        if (blockIdx.x == 0) {
+          st.sc[2] = lim.x;
+          st.sc[3] = lim.y;
+          st.sc[4] = lim.z;
+          st.sc[5] = lim.w;
+        }
+      }
+    }
+    __syncthreads();
+    base = blockIdx.x * kTileItems;
+    end = s_rng.x;
+  } else if constexpr (kRows) {
+    const uint4 r = row_table(st);
     const uint32_t b = blockIdx.x;
     if (r.w > 0u && b >= r.z && b - r.z < r.w) {  // one row matches at most
       const uint32_t lo = r.x + (b - r.z) * kTileItems;
       s_rng = make_uint4(lo, min(lo + kTileItems, r.y), r.z, threadIdx.x | (r.w << 8));
     }
+    if (b == gridDim.x - 1 && (int)threadIdx.x < st.gy && r.w == 0u)
+      for (int x = 0; x < st.gx; ++x) {
+        const int64_t t = (int64_t)threadIdx.x * st.gx + x;
+        st.tile_range[2 * t] = 0;
+        st.tile_range[2 * t + 1] = 0;
+      }
     __syncthreads();
     const uint4 bi = s_rng;
     if (threadIdx.x == 0) binfo[b] = bi;
@@ -1576,67 +1679,6 @@ struct LoadSegW {
   __device__ uint32_t operator()(uint32_t i) const { return keys[i] >> 16; }
 };
 
-// Segments of the first Mc = min(M, m) instances of the depth-first sequence
-// (an instance capacity m below the count M keeps that prefix): sc[0] = Nv,
-// sc[1] = M, sc[6] = S (all segments); writes sc[2] = the segments kept,
-// sc[3] = the last kept one if the capacity cuts it short (else 0xFFFFFFFF)
-// and sc[4] its width, sc[5] = Mc.  One wave.
-__global__ __launch_bounds__(64) void seg_limit_kernel(const uint32_t *__restrict__ offs,
-                                                       const uint32_t *__restrict__ soffs,
-                                                       const uint2 *__restrict__ rects,
-                                                       uint32_t cap_n, uint32_t m,
-                                                       uint32_t *sc) {
-  const uint32_t nv = min(sc[0], cap_n), M = sc[1];
-  const uint32_t mc = min(M, m);
-  uint32_t seff = sc[6], qfix = 0xFFFFFFFFu, wfix = 0u;
-  if (M > m) {  // uniform
-    seff = 0u;
-    if (mc > 0u && nv > 0u) {
-      const uint32_t q = mc - 1u;  // the last kept instance
-      const uint32_t s = find_owner(offs, nv, q);
-      const uint32_t w = max(rects[s].y & 0xFFFFu, 1u);
-      const uint32_t k = q - offs[s];
-      const uint32_t row = k / w;
-      seff = soffs[s] + row + 1u;
-      qfix = seff - 1u;
-      wfix = k - row * w + 1u;
-    }
-  }
-  if (lane_id() == 0) {
-    sc[2] = seff;
-    sc[3] = qfix;
-    sc[4] = wfix;
-    sc[5] = mc;
-  }
-}
-
-// Row table of pass B from pass A's digit totals (segments per row) and E:
-// rows[ty] = {first instance, end, first block, block count} (4096 instances
-// per block, blocks numbered row after row); the tiles of rows without
-// instances get (0, 0) here (the others from pass B's first block of the row).
-__global__ __launch_bounds__(kThreads) void rows_kernel(const uint32_t *__restrict__ seg_per_row,
-                                                        const uint32_t *__restrict__ E,
-                                                        const uint32_t *__restrict__ sc, int gx,
-                                                        int gy, uint4 *__restrict__ rows,
-                                                        int32_t *__restrict__ tile_range) {
-  const int ty = threadIdx.x;
-  const uint32_t seff = sc[2], mc = sc[5];
-  const uint32_t cnt = ty < gy ? seg_per_row[ty] : 0u;
-  uint32_t total;
-  const uint32_t seg0 = block_exclusive_scan(cnt, total);
-  const uint32_t seg1 = seg0 + cnt;
-  const uint32_t e0 = seg0 < seff ? E[seg0] : mc;
-  const uint32_t e1 = seg1 < seff ? E[seg1] : mc;
-  const uint32_t nbr = (e1 - e0 + kTileItems - 1) / kTileItems;
-  const uint32_t blk0 = block_exclusive_scan(nbr, total);
-  rows[ty] = make_uint4(e0, e1, blk0, nbr);
-  if (ty < gy && nbr == 0u)
-    for (int x = 0; x < gx; ++x) {
-      tile_range[2 * ((int64_t)ty * gx + x)] = 0;
-      tile_range[2 * ((int64_t)ty * gx + x) + 1] = 0;
-    }
-}
-
 // --------------------------------------------------------------------------
 // Host-side plumbing
 // --------------------------------------------------------------------------
@@ -1784,7 +1826,7 @@ bool row_binning(const Grid &gr, int T, int64_t n) {
 
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
-  // scalars[0] = Nv, scalars[1] = M; row binning: [2..5] (seg_limit_kernel), [6] = S
+  // scalars[0] = Nv, scalars[1] = M; row binning: [2..5] (seg_limit), [6] = S
   uint32_t *cnt, *offs, *scalars;
   uint32_t *soffs;                 // row binning: segment offsets, in depth order
   uint32_t *bsum;                  // instances per count block
@@ -1829,9 +1871,9 @@ size_t gsr_sort_ws_inst_bytes(int64_t m) {
   return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8) +
          align256(nb * (kThreads / 64) * 4) + align256(nb * 2 * kRadix * 4) +
          align256(kRadix * 4) + 2 * align256((uint64_t)kCountTiles * 4) +
-         // row binning: pass B's first[], histogram, totals, binfo; the row table
+         // row binning: pass B's first[], histogram, totals, binfo
          align256(nbr * (kThreads / 64) * 4) + align256(nbr * kRadix * 4) +
-         align256(kRadix * 4) + align256(nbr * 16) + align256(kRadix * 16);
+         align256(kRadix * 4) + align256(nbr * 16);
 }
 
 int gsr_set_binning_mode(int mode) {
@@ -1898,20 +1940,27 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     const int rc = copy_count_to_host(m_out, m_host, event, s);
     return rc != GSR_OK ? rc : gsr_launch_status("sort_count");
   }
-  const uint32_t ncb = (uint32_t)gsr_blocks(n, kThreads);
-  if (gr.conic == nullptr)
-    count_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.rects, g.bsum);
-  else
-    count_tight_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.masks,
-                                                g.bsum);
   // compact visible splats with their depth bits as keys -> Nv in scalars[0];
   // the same launch sums the count blocks' totals -> M in m_out
   const bool exact = gr.conic == nullptr;
-  run_scan(LoadFlag{g.cnt},
-           StoreCompact{depth, g.sb.keyA, g.sb.valA, exact ? g.rects : nullptr,
-                        reinterpret_cast<uint2 *>(g.masks)},
-           nullptr, (uint32_t)n, g.sb.partials, TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out},
-           s);
+  if (exact) {
+    // the count kernel writes the compaction's block sums itself
+    const uint32_t nb = nblk((uint64_t)n);
+    count_kernel<<<nb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.rects, g.bsum,
+                                         g.sb.partials);
+    scan_down_kernel<<<nb, kThreads, 0, s>>>(
+        LoadFlag{g.cnt},
+        StoreCompact{depth, g.sb.keyA, g.sb.valA, g.rects, reinterpret_cast<uint2 *>(g.masks)},
+        nullptr, (uint32_t)n, g.sb.partials, nb,
+        TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out});
+  } else {
+    const uint32_t ncb = (uint32_t)gsr_blocks(n, kThreads);
+    count_tight_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.masks,
+                                                g.bsum);
+    run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA, nullptr, nullptr},
+             nullptr, (uint32_t)n, g.sb.partials, TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out},
+             s);
+  }
   {
     const int rc = copy_count_to_host(m_out, m_host, event, s);
     if (rc != GSR_OK) return rc;
@@ -1975,34 +2024,30 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
   if (row_binning(gr, T, n)) {
     const uint32_t nbB = row_blocks((uint64_t)m);
     uint32_t *firstB, *histB, *totalsB;
-    uint4 *binfo, *rows;
+    uint4 *binfo;
     p = carve(p, (uint64_t)nbB * (kThreads / 64) * 4, (void **)&firstB);
     p = carve(p, (uint64_t)nbB * kRadix * 4, (void **)&histB);
     p = carve(p, (uint64_t)kRadix * 4, (void **)&totalsB);
-    p = carve(p, (uint64_t)nbB * 16, (void **)&binfo);
-    carve(p, (uint64_t)kRadix * 16, (void **)&rows);
+    carve(p, (uint64_t)nbB * 16, (void **)&binfo);
     const uint32_t mu = (uint32_t)m;
     const int bitsA = tile_bits(gr.gy), bitsB = tile_bits(gr.gx);
     const uint2 *rects = reinterpret_cast<const uint2 *>(g.masks);
-    seg_limit_kernel<<<1, 64, 0, s>>>(g.offs, g.soffs, rects, (uint32_t)n, mu, g.scalars);
     // pass A: segments of the depth-ordered splats, stably by row
-    const RowStream rs{g.sb.valA, g.soffs, rects};
-    gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, g.scalars + 2,
-                                                       mu, gr, (1u << bitsA) - 1u, ib.hist, nb,
-                                                       first, nullptr, nullptr);
+    const RowStream rs{g.sb.valA, g.soffs, rects, g.offs, g.scalars, mu};
+    gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, nullptr, mu,
+                                                       gr, (1u << bitsA) - 1u, ib.hist, nb,
+                                                       first, nullptr);
     digit_scan_kernel<<<1u << bitsA, kThreads, 0, s>>>(ib.hist, nb, totals1);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
         ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, ib.hist, nb, totals1, ScatterExtras{});
     // E: each segment's first instance in pass A's order
     run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, TotNone{}, s);
-    rows_kernel<<<1, kThreads, 0, s>>>(totals1, ib.keyB, g.scalars, gr.gx, gr.gy, rows,
-                                       tile_range);
     // pass B: each row's instances, stably by column, straight to ids_out
-    const SegStream ss{ib.valA, ib.keyB, ib.keyA};
+    const SegStream ss{ib.valA, ib.keyB, ib.keyA, totals1, g.scalars, gr.gx, gr.gy, tile_range};
     gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
         ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
-        rows, binfo);
+        binfo);
     digit_scan_kernel<<<1u << bitsB, kThreads, 0, s>>>(histB, nbB, totalsB);
     ScatterExtras ex;
     ex.binfo = binfo;
@@ -2054,7 +2099,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     gen_hist_kernel<SplatStream><<<nb, kThreads, 0, s>>>(st, g.scalars, (uint32_t)n,
                                                          g.scalars + 1, (uint32_t)m, gr,
                                                          (1u << bits0) - 1u, ib.hist, nb, first,
-                                                         nullptr, nullptr);
+                                                         nullptr);
     digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, totals1);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<SplatStream>{st, g.scalars, (uint32_t)n, gr, first, nullptr},
