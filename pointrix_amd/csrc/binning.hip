@@ -34,6 +34,27 @@ __device__ __forceinline__ uint32_t count_of(const uint32_t *n_ptr, uint32_t cap
   return n_ptr ? min(*n_ptr, cap) : cap;
 }
 
+// Logical block index of the radix kernels.  Workgroups go to the 8 XCDs
+// round-robin (workgroup i to XCD i % 8), each XCD with its own L2.  Blocks
+// whose writes share cache lines -- neighbouring columns of a digit's
+// histogram row, the line where one block's output run of a digit ends and
+// the next block's begins -- get consecutive logical indices on one XCD, so
+// their partial-line writes merge in one L2 instead of leaving eight L2s as
+// separate partial writes.  Only the first na blocks hold items (grids are
+// sized for a capacity, the item count is on the device): those are spread
+// over all eight XCDs, the rest keep their index.  A bijection of
+// [0, gridDim.x); the histogram and scatter kernels of a pass compute it from
+// the same count.
+__device__ __forceinline__ uint32_t logical_block(uint32_t na) {
+  const uint32_t b = blockIdx.x;
+  if (b >= na) return b;
+  const uint32_t q = na >> 3, r = na & 7u, x = b & 7u, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+__device__ __forceinline__ uint32_t blocks_of(uint32_t n) {
+  return (n + kTileItems - 1) / kTileItems;
+}
+
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
   __shared__ uint32_t s_w[kThreads / 64];
@@ -265,7 +286,8 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   s_t[1][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = count_of(n_ptr, cap);
-  const uint32_t base = blockIdx.x * kTileItems;
+  const uint32_t lb = logical_block(blocks_of(n));
+  const uint32_t base = lb * kTileItems;
   const bool tiles = tc.tcnt != nullptr;
   const uint32_t lmask = (1u << shift) - 1u;
   const uint32_t lf = (tiles && base < n) ? (keys[base] & lmask) : 0u;
@@ -306,11 +328,11 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
     }
   }
   __syncthreads();
-  hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+  hist[threadIdx.x * nb + lb] = s_h[threadIdx.x];
   if (tiles) {
-    tc.tcnt[(2 * (size_t)blockIdx.x) * kRadix + threadIdx.x] =
+    tc.tcnt[(2 * (size_t)lb) * kRadix + threadIdx.x] =
         split ? s_t[0][threadIdx.x] : s_h[threadIdx.x];
-    tc.tcnt[(2 * (size_t)blockIdx.x + 1) * kRadix + threadIdx.x] =
+    tc.tcnt[(2 * (size_t)lb + 1) * kRadix + threadIdx.x] =
         split ? s_t[1][threadIdx.x] : 0u;
   }
 }
@@ -510,6 +532,7 @@ struct ScatterExtras {
   uint2 *pout = nullptr;
   bool kout_area = false;
   const uint4 *binfo = nullptr;
+  const uint32_t *row_blocks = nullptr;  // with binfo: the number of row blocks
   int32_t *tile_range = nullptr;
   int gx = 0;
 };
@@ -531,15 +554,17 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   __shared__ int32_t s_val[kPk ? 1 : kTileItems];
   __shared__ uint16_t s_src[kPay ? kTileItems : 1];  // block-local input index
   __shared__ uint32_t s_flag[kThreads];
-  uint32_t n, base;
+  uint32_t n, base, lb;
   uint4 bi = make_uint4(0u, 0u, 0u, 0u);
   if (ex.binfo != nullptr) {
-    bi = ex.binfo[blockIdx.x];
+    lb = logical_block(*ex.row_blocks);
+    bi = ex.binfo[lb];
     base = bi.x;
     n = bi.y;
   } else {
     n = count_of(n_ptr, cap);
-    base = blockIdx.x * kTileItems;
+    lb = logical_block(blocks_of(n));
+    base = lb * kTileItems;
   }
   if (base >= n) return;  // uniform over the block
   const uint32_t mask = (1u << bits) - 1u;
@@ -554,14 +579,14 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   // this thread's digit total and block offset, requested with the items
   const bool my_digit = (int)threadIdx.x < (1 << bits);
   const uint32_t my_total = my_digit ? totals[threadIdx.x] : 0u;
-  const uint32_t my_hist = hist_scanned[threadIdx.x * nb + blockIdx.x];
+  const uint32_t my_hist = hist_scanned[threadIdx.x * nb + lb];
   // row blocks: this column's count before the row's first block and after its last
   uint32_t row_p0 = 0, row_p1 = 0;
   if (ex.binfo != nullptr && my_digit) {
     row_p0 = hist_scanned[threadIdx.x * nb + bi.z];
     row_p1 = hist_scanned[threadIdx.x * nb + bi.z + (bi.w >> 8)];
   }
-  items.load(blockIdx.x, base, wbase, n, key, val, s_key, s_val, s_flag);
+  items.load(lb, base, wbase, n, key, val, s_key, s_val, s_flag);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
@@ -608,10 +633,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       // then d in the row's earlier blocks
       const uint32_t blk0 = bi.z, ty = bi.w & 0xFFu;
       const uint32_t c = row_p1 - row_p0;
-      const uint32_t e0 = base - (blockIdx.x - blk0) * kTileItems;
+      const uint32_t e0 = base - (lb - blk0) * kTileItems;
       const uint32_t tstart = e0 + block_exclusive_scan(c, total);
       s_delta[d] = (int32_t)(tstart + (my_hist - row_p0) - lstart);
-      if (blockIdx.x == blk0 && d < ex.gx) {
+      if (lb == blk0 && d < ex.gx) {
         const int64_t t = (int64_t)ty * ex.gx + d;
         ex.tile_range[2 * t] = c ? (int32_t)tstart : 0;
         ex.tile_range[2 * t + 1] = c ? (int32_t)(tstart + c) : 0;
@@ -1190,6 +1215,7 @@ struct SegStream {
   const uint32_t *sc;           // scalars: [2] segments kept, [5] instances kept
   int gx, gy;
   int32_t *tile_range;          // the tiles of rows without instances get (0, 0)
+  uint32_t *nblocks;            // receives the number of row blocks
 };
 
 __device__ __forceinline__ EmitWave stream_load(const SegStream &st, uint32_t nv, const Grid &,
@@ -1347,8 +1373,9 @@ __device__ __forceinline__ uint4 seg_limit(const uint32_t *__restrict__ offs,
 // Pass B's row table, one row per thread of the block: row ty's segments are
 // [seg0, seg1) of pass A's output (prefix of the digit totals), its instances
 // [E[seg0], E[seg1]) (Mc past the last segment), cut into ceil(count / 4096)
-// blocks numbered row after row from blk0.  -> {e0, e1, blk0, block count}
-__device__ __forceinline__ uint4 row_table(const SegStream &st) {
+// blocks numbered row after row from blk0.  -> {e0, e1, blk0, block count};
+// nblocks = the blocks of all rows.
+__device__ __forceinline__ uint4 row_table(const SegStream &st, uint32_t &nblocks) {
   const int ty = threadIdx.x;
   const uint32_t seff = st.sc[2], mc = st.sc[5];
   const uint32_t cnt = ty < st.gy ? st.seg_per_row[ty] : 0u;
@@ -1358,7 +1385,7 @@ __device__ __forceinline__ uint4 row_table(const SegStream &st) {
   const uint32_t e0 = seg0 < seff ? st.offs[seg0] : mc;
   const uint32_t e1 = seg1 < seff ? st.offs[seg1] : mc;
   const uint32_t nbr = (e1 - e0 + kTileItems - 1) / kTileItems;
-  const uint32_t blk0 = block_exclusive_scan(nbr, total);
+  const uint32_t blk0 = block_exclusive_scan(nbr, nblocks);
   return make_uint4(e0, e1, blk0, nbr);
 }
 
@@ -1397,7 +1424,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     s_rng = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
-  uint32_t base, end;
+  uint32_t base, end, lb;
   if constexpr (Stream::kLimit) {
     if (threadIdx.x < 64) {  // wave 0
       const uint4 lim = seg_limit(st.ioffs, st.offs, st.rects, cap_n, st.m, st.sc);
@@ -1412,16 +1439,20 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
       }
     }
     __syncthreads();
-    base = blockIdx.x * kTileItems;
     end = s_rng.x;
+    lb = logical_block(blocks_of(end));
+    base = lb * kTileItems;
   } else if constexpr (kRows) {
-    const uint4 r = row_table(st);
-    const uint32_t b = blockIdx.x;
+    uint32_t nrb;
+    const uint4 r = row_table(st, nrb);
+    lb = logical_block(nrb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.nblocks[0] = nrb;
+    const uint32_t b = lb;
     if (r.w > 0u && b >= r.z && b - r.z < r.w) {  // one row matches at most
       const uint32_t lo = r.x + (b - r.z) * kTileItems;
       s_rng = make_uint4(lo, min(lo + kTileItems, r.y), r.z, threadIdx.x | (r.w << 8));
     }
-    if (b == gridDim.x - 1 && (int)threadIdx.x < st.gy && r.w == 0u)
+    if (blockIdx.x == gridDim.x - 1 && (int)threadIdx.x < st.gy && r.w == 0u)
       for (int x = 0; x < st.gx; ++x) {
         const int64_t t = (int64_t)threadIdx.x * st.gx + x;
         st.tile_range[2 * t] = 0;
@@ -1433,8 +1464,9 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     base = bi.x;
     end = bi.y;
   } else {
-    base = blockIdx.x * kTileItems;
     end = count_of(m_ptr, cap_m);
+    lb = logical_block(blocks_of(end));
+    base = lb * kTileItems;
   }
   const uint32_t nv = count_of(nv_ptr, cap_n);
   const int w = threadIdx.x >> 6;
@@ -1445,7 +1477,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   if (q0 < end) {  // uniform per wave
     const uint32_t q1 = min(q0 + kWaveItems, end);
     const uint32_t s0 = find_owner(st.offs, nv, q0);
-    if (lane_id() == 0) first[blockIdx.x * (kThreads / 64) + w] = s0;
+    if (lane_id() == 0) first[lb * (kThreads / 64) + w] = s0;
     EmitWave cur = stream_load(st, nv, g, s0);
     for (uint32_t sb = s0;; sb += 64) {
       // the next group is requested before this one is counted
@@ -1488,7 +1520,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   const int32_t v = threadIdx.x < D ? s_d[threadIdx.x] : 0;
   uint32_t total;
   const uint32_t excl = block_exclusive_scan((uint32_t)v, total);
-  hist[threadIdx.x * nb + blockIdx.x] = threadIdx.x < D ? s_full + excl + (uint32_t)v : 0u;
+  hist[threadIdx.x * nb + lb] = threadIdx.x < D ? s_full + excl + (uint32_t)v : 0u;
 }
 
 // Items of a radix block generated from a stream (the first tile pass: the
@@ -1826,7 +1858,8 @@ bool row_binning(const Grid &gr, int T, int64_t n) {
 
 // Gaussian-level workspace: cnt, offsets, scalars + sort buffers.
 struct GaussWs {
-  // scalars[0] = Nv, scalars[1] = M; row binning: [2..5] (seg_limit), [6] = S
+  // scalars[0] = Nv, scalars[1] = M; row binning: [2..5] (seg_limit), [6] = S,
+  // [7] = pass B's row blocks
   uint32_t *cnt, *offs, *scalars;
   uint32_t *soffs;                 // row binning: segment offsets, in depth order
   uint32_t *bsum;                  // instances per count block
@@ -2044,13 +2077,15 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     // E: each segment's first instance in pass A's order
     run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, TotNone{}, s);
     // pass B: each row's instances, stably by column, straight to ids_out
-    const SegStream ss{ib.valA, ib.keyB, ib.keyA, totals1, g.scalars, gr.gx, gr.gy, tile_range};
+    const SegStream ss{ib.valA,  ib.keyB, ib.keyA,    totals1,
+                       g.scalars, gr.gx, gr.gy, tile_range, g.scalars + 7};
     gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
         ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
         binfo);
     digit_scan_kernel<<<1u << bitsB, kThreads, 0, s>>>(histB, nbB, totalsB);
     ScatterExtras ex;
     ex.binfo = binfo;
+    ex.row_blocks = g.scalars + 7;
     ex.tile_range = tile_range;
     ex.gx = gr.gx;
     radix_scatter_kernel<<<nbB, kThreads, 0, s>>>(
