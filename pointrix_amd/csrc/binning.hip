@@ -288,6 +288,7 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   const uint32_t n = count_of(n_ptr, cap);
   const uint32_t lb = logical_block(blocks_of(n));
   const uint32_t base = lb * kTileItems;
+  if (base >= n) return;  // no items: no column (the digit scan covers the others)
   const bool tiles = tc.tcnt != nullptr;
   const uint32_t lmask = (1u << shift) - 1u;
   const uint32_t lf = (tiles && base < n) ? (keys[base] & lmask) : 0u;
@@ -434,15 +435,29 @@ __global__ __launch_bounds__(1024) void ranges_from_counts_kernel(
   }
 }
 
+// Histogram columns a digit scan covers: of the nb a grid has, those of the
+// blocks that hold items, ceil(min(*p, cap) / div) + add (p == nullptr: all);
+// the blocks past them write no column.
+struct Cols {
+  const uint32_t *p = nullptr;
+  uint32_t cap = 0, div = kTileItems, add = 0;
+  __device__ uint32_t of(uint32_t nb) const {
+    return p ? min(nb, (min(*p, cap) + div - 1) / div + add) : nb;
+  }
+};
+
 // Per digit d (one block each): hist[d][b] <- sum of hist[d][b'] over
-// b' < b (exclusive, in place) and totals[d] <- the digit's total.  Thread t
-// owns a contiguous run of blocks; run sums are scanned across the block.
+// b' < b (exclusive, in place) and totals[d] <- the digit's total, over the
+// columns cols covers (rows of nb).  Thread t owns a contiguous run of blocks;
+// run sums are scanned across the block.
 __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t *__restrict__ hist,
                                                               uint32_t nb,
-                                                              uint32_t *__restrict__ totals) {
+                                                              uint32_t *__restrict__ totals,
+                                                              Cols cols) {
   uint32_t *row = hist + (size_t)blockIdx.x * nb;
-  const uint32_t per = (nb + kThreads - 1) / kThreads;
-  const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+  const uint32_t nc = cols.of(nb);
+  const uint32_t per = (nc + kThreads - 1) / kThreads;
+  const uint32_t b0 = min(threadIdx.x * per, nc), b1 = min(b0 + per, nc);
   constexpr uint32_t kHeld = 32;
   if (per <= kHeld) {
     // the run held in registers: one read (all loads in flight), one write
@@ -1424,7 +1439,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     s_rng = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
-  uint32_t base, end, lb;
+  uint32_t base, end, lb, rows_end = 0;
   if constexpr (Stream::kLimit) {
     if (threadIdx.x < 64) {  // wave 0
       const uint4 lim = seg_limit(st.ioffs, st.offs, st.rects, cap_n, st.m, st.sc);
@@ -1446,6 +1461,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     uint32_t nrb;
     const uint4 r = row_table(st, nrb);
     lb = logical_block(nrb);
+    rows_end = nrb;
     if (blockIdx.x == 0 && threadIdx.x == 0) st.nblocks[0] = nrb;
     const uint32_t b = lb;
     if (r.w > 0u && b >= r.z && b - r.z < r.w) {  // one row matches at most
@@ -1516,6 +1532,9 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     }
   }
   __syncthreads();
+  // blocks without items write no column, except in row mode the one right
+  // after the last row block (zeros: it closes the row counts)
+  if (kRows ? base >= end && lb != rows_end : base >= end) return;  // uniform
   // counts = the cycles every digit got + the prefix of the difference array
   const int32_t v = threadIdx.x < D ? s_d[threadIdx.x] : 0;
   uint32_t total;
@@ -1794,7 +1813,7 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     uint32_t *totals = (p == 0 && o.totals0 != nullptr) ? o.totals0 : b.totals;
     radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb,
                                               last ? o.final_tc : TileCount{});
-    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals);
+    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals, Cols{n_ptr, cap});
     ScatterExtras ex;
     ex.pin = o.pay_cur;
     ex.pout = o.pay_alt;
@@ -2070,7 +2089,8 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, nullptr, mu,
                                                        gr, (1u << bitsA) - 1u, ib.hist, nb,
                                                        first, nullptr);
-    digit_scan_kernel<<<1u << bitsA, kThreads, 0, s>>>(ib.hist, nb, totals1);
+    digit_scan_kernel<<<1u << bitsA, kThreads, 0, s>>>(ib.hist, nb, totals1,
+                                                       Cols{g.scalars + 2, mu});
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
         ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, ib.hist, nb, totals1, ScatterExtras{});
@@ -2082,7 +2102,8 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
         ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
         binfo);
-    digit_scan_kernel<<<1u << bitsB, kThreads, 0, s>>>(histB, nbB, totalsB);
+    digit_scan_kernel<<<1u << bitsB, kThreads, 0, s>>>(histB, nbB, totalsB,
+                                                       Cols{g.scalars + 7, nbB, 1, 1});
     ScatterExtras ex;
     ex.binfo = binfo;
     ex.row_blocks = g.scalars + 7;
@@ -2135,7 +2156,8 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
                                                          g.scalars + 1, (uint32_t)m, gr,
                                                          (1u << bits0) - 1u, ib.hist, nb, first,
                                                          nullptr);
-    digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, totals1);
+    digit_scan_kernel<<<1u << bits0, kThreads, 0, s>>>(ib.hist, nb, totals1,
+                                                       Cols{g.scalars + 1, (uint32_t)m});
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<SplatStream>{st, g.scalars, (uint32_t)n, gr, first, nullptr},
         (passes == 1 && by_counts) ? nullptr : ib.keyA, v_pass0, g.scalars + 1, (uint32_t)m, 0,
