@@ -550,6 +550,13 @@ struct ScatterExtras {
   const uint32_t *row_blocks = nullptr;  // with binfo: the number of row blocks
   int32_t *tile_range = nullptr;
   int gx = 0;
+  // e_out (row binning's pass A): each item's first instance in the sorted
+  // order, E = the digit's (row's) first instance + the widths (key >> 16) of
+  // the digit's items before it: whist = the scanned width histogram rows,
+  // wtot = the width totals per digit
+  uint32_t *e_out = nullptr;
+  const uint32_t *whist = nullptr;
+  const uint32_t *wtot = nullptr;
 };
 
 // kPay: the pass carries a payload (ex.pin / ex.pout); its LDS index array
@@ -600,6 +607,12 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   if (ex.binfo != nullptr && my_digit) {
     row_p0 = hist_scanned[threadIdx.x * nb + bi.z];
     row_p1 = hist_scanned[threadIdx.x * nb + bi.z + (bi.w >> 8)];
+  }
+  // E: this digit's widths in earlier blocks and in all blocks
+  uint32_t my_w = 0, my_wt = 0;
+  if (ex.e_out != nullptr && my_digit) {
+    my_w = ex.whist[threadIdx.x * nb + lb];
+    my_wt = ex.wtot[threadIdx.x];
   }
   items.load(lb, base, wbase, n, key, val, s_key, s_val, s_flag);
   __syncthreads();
@@ -689,6 +702,65 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
                                  : (ex.kgather != nullptr ? ex.kgather[vv] : kk);
       vout[pos] = vv;
       if (ex.rank_out != nullptr) ex.rank_out[vv] = pos;
+    }
+  }
+  if constexpr (!kPk) {
+    if (ex.e_out != nullptr) {  // uniform
+      // the block's items in LDS (digit) order: wave w scans the widths of
+      // items [1024 w, 1024 w + 1024) in rows of 64 (lane = item, no bank
+      // conflicts) into each item's exclusive width prefix within the wave,
+      // kept in s_val (read above, free now); s_woff = the waves' offsets
+      __shared__ uint32_t s_woff[kThreads / 64];
+      __syncthreads();
+      const int wv = threadIdx.x >> 6, ln = lane_id();
+      uint32_t carry = 0;
+#pragma unroll 4
+      for (int r = 0; r < kItems; ++r) {
+        const uint32_t idx = (uint32_t)wv * (kTileItems / 4) + r * 64 + ln;
+        const uint32_t wd = idx < cnt ? s_key[idx] >> 16 : 0u;
+        uint32_t inc = wd;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const uint32_t t = __shfl_up(inc, d, 64);
+          if (ln >= d) inc += t;
+        }
+        if (idx < cnt) s_val[idx] = (int32_t)(carry + inc - wd);
+        carry += (uint32_t)__shfl((int)inc, 63, 64);
+      }
+      if (ln == 0) s_woff[wv] = carry;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < kThreads / 64; ++k) {
+          const uint32_t t = s_woff[k];
+          s_woff[k] = run;
+          run += t;
+        }
+      }
+      __syncthreads();
+      uint32_t tot;
+      {
+        // digit d's first instance in this block: the digit's first instance
+        // overall (prefix of the totals) + its widths in earlier blocks, less
+        // the width prefix at the digit's first LDS slot (s_cnt[0][d]; a
+        // digit without items here is never looked up)
+        const int d = threadIdx.x;
+        const uint32_t dstart = block_exclusive_scan(my_wt, tot);
+        const uint32_t ls = s_cnt[0][d];
+        const uint32_t gls =
+            ls < cnt ? (uint32_t)s_val[ls] + s_woff[ls / (kTileItems / 4)] : 0u;
+        s_flag[d] = dstart + my_w - gls;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int k = 0; k < kItems; ++k) {
+        const uint32_t i = k * kThreads + threadIdx.x;
+        if (i < cnt) {
+          const uint32_t d = (s_key[i] >> shift) & mask;
+          ex.e_out[i + (uint32_t)s_delta[d]] =
+              s_flag[d] + (uint32_t)s_val[i] + s_woff[i / (kTileItems / 4)];
+        }
+      }
     }
   }
 }
@@ -1170,8 +1242,10 @@ __device__ __forceinline__ EmitWave stream_load(const SplatStream &st, uint32_t 
 // the splat id -- stably by the row ty; pass B expands every segment into its
 // w tiles and sorts them stably by the column tx, each radix block inside one
 // row, so the row-major tile order and the depth order inside a tile follow.
-// Pass A moves S ~ M / 4 items, and pass B writes the final ids and the tile
-// ranges: no pass reads or writes the instance keys.
+// Pass A moves S ~ M / 4 items and also writes each segment's first instance
+// E (its histogram counts the widths per row beside the segments), and pass B
+// writes the final ids and the tile ranges: no pass reads or writes the
+// instance keys.
 //
 // Both passes reuse the generator of the first tile pass: an element with n
 // items is a one-row rectangle of width n (inv_w = 1 / n, row 0), and the
@@ -1217,8 +1291,8 @@ __device__ __forceinline__ EmitWave stream_load(const RowStream &st, uint32_t nv
 }
 
 // Pass B's elements are pass A's output in order: segment s (key, splat id)
-// starting at instance E[s] (the exclusive prefix of the widths); item k =
-// column x0 + k.
+// starting at instance E[s] (the exclusive prefix of the widths in that
+// order, written by pass A's scatter); item k = column x0 + k.
 struct SegStream {
   static constexpr bool kOneRow = true;
   static constexpr bool kLimit = false;
@@ -1429,13 +1503,18 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     Stream st, const uint32_t *nv_ptr, uint32_t cap_n, const uint32_t *m_ptr,
     uint32_t cap_m, Grid g, uint32_t mask, uint32_t *__restrict__ hist, uint32_t nb,
     uint32_t *__restrict__ first, uint4 *__restrict__ binfo) {
+  constexpr bool kW = Stream::kLimit;  // pass A: also the widths per row
   __shared__ int32_t s_d[kRadix + 1];
-  __shared__ uint32_t s_full;
+  __shared__ int32_t s_dw[kW ? kRadix + 1 : 1];
+  __shared__ uint32_t s_full, s_wfull;
   __shared__ uint4 s_rng;
   s_d[threadIdx.x] = 0;
+  if constexpr (kW) s_dw[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     s_d[kRadix] = 0;
+    if constexpr (kW) s_dw[kRadix] = 0;
     s_full = 0;
+    s_wfull = 0;
     s_rng = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
@@ -1525,6 +1604,24 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
               atomicAdd(&s_d[d1 - D], -1);
             }
           }
+          if constexpr (kW) {
+            // the same run weighted by the splat's tile width (rows: no wrap,
+            // a splat covers at most the D >= gy rows once)
+            const int32_t wt = (int32_t)((uint32_t)cur.toff >> 16);
+            if (full) atomicAdd(&s_wfull, (uint32_t)wt);
+            if (rem) {
+              const uint32_t d0 = t0 & mask;
+              atomicAdd(&s_dw[d0], wt);
+              atomicAdd(&s_dw[d0 + rem], -wt);
+            }
+            // the segment a capacity cuts short has width wfix, not wt
+            const uint32_t qf = s_rng.y;
+            if (qf >= a && qf < b) {
+              const uint32_t dq = (t0 + (qf - cur.off - (uint32_t)k0)) & mask;
+              atomicAdd(&s_dw[dq], (int32_t)s_rng.z - wt);
+              atomicAdd(&s_dw[dq + 1], wt - (int32_t)s_rng.z);
+            }
+          }
         }
       }
       if (cur.r1 >= q1 || sb + 64 >= nv) break;  // uniform
@@ -1540,6 +1637,12 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   uint32_t total;
   const uint32_t excl = block_exclusive_scan((uint32_t)v, total);
   hist[threadIdx.x * nb + lb] = threadIdx.x < D ? s_full + excl + (uint32_t)v : 0u;
+  if constexpr (kW) {
+    // rows D .. 2D - 1: the widths (instances) per row
+    const int32_t vw = threadIdx.x < D ? s_dw[threadIdx.x] : 0;
+    const uint32_t ew = block_exclusive_scan((uint32_t)vw, total);
+    if (threadIdx.x < D) hist[(D + threadIdx.x) * nb + lb] = s_wfull + ew + (uint32_t)vw;
+  }
 }
 
 // Items of a radix block generated from a stream (the first tile pass: the
@@ -1722,12 +1825,6 @@ struct StoreOffs2 {
     offs[i] = excl.x;
     soffs[i] = excl.y;
   }
-};
-// Width of each segment in pass A's output, scanned into E.
-struct LoadSegW {
-  using value_type = uint32_t;
-  const uint32_t *keys;
-  __device__ uint32_t operator()(uint32_t i) const { return keys[i] >> 16; }
 };
 
 // --------------------------------------------------------------------------
@@ -2084,20 +2181,26 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     const uint32_t mu = (uint32_t)m;
     const int bitsA = tile_bits(gr.gy), bitsB = tile_bits(gr.gx);
     const uint2 *rects = reinterpret_cast<const uint2 *>(g.masks);
-    // pass A: segments of the depth-ordered splats, stably by row
+    // pass A: segments of the depth-ordered splats, stably by row; its
+    // histogram rows [0, D) count segments and rows [D, 2D) their widths (in
+    // tcnt, 2 x 256 rows), the digit totals go to counts ([0, D) segments per
+    // row, [D, 2D) instances per row), and its scatter writes each segment's
+    // first instance E
+    const uint32_t DA = 1u << bitsA;
+    uint32_t *histA = tcnt, *totA = counts;
     const RowStream rs{g.sb.valA, g.soffs, rects, g.offs, g.scalars, mu};
     gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, nullptr, mu,
-                                                       gr, (1u << bitsA) - 1u, ib.hist, nb,
-                                                       first, nullptr);
-    digit_scan_kernel<<<1u << bitsA, kThreads, 0, s>>>(ib.hist, nb, totals1,
-                                                       Cols{g.scalars + 2, mu});
+                                                       gr, DA - 1u, histA, nb, first, nullptr);
+    digit_scan_kernel<<<2 * DA, kThreads, 0, s>>>(histA, nb, totA, Cols{g.scalars + 2, mu});
+    ScatterExtras exA;
+    exA.e_out = ib.keyB;
+    exA.whist = histA + (size_t)DA * nb;
+    exA.wtot = totA + DA;
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
-        ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, ib.hist, nb, totals1, ScatterExtras{});
-    // E: each segment's first instance in pass A's order
-    run_scan(LoadSegW{ib.keyA}, StoreU32{ib.keyB}, g.scalars + 2, mu, ib.partials, TotNone{}, s);
+        ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, histA, nb, totA, exA);
     // pass B: each row's instances, stably by column, straight to ids_out
-    const SegStream ss{ib.valA,  ib.keyB, ib.keyA,    totals1,
+    const SegStream ss{ib.valA,  ib.keyB, ib.keyA,    totA,
                        g.scalars, gr.gx, gr.gy, tile_range, g.scalars + 7};
     gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
         ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
