@@ -559,8 +559,9 @@ struct ScatterExtras {
   const uint32_t *wtot = nullptr;
 };
 
-// kPay: the pass carries a payload (ex.pin / ex.pout); its LDS index array
-// exists only then, so the other passes keep their occupancy.
+// kPay: the pass carries an 8-byte payload (ex.pin / ex.pout), loaded with
+// the items and staged through LDS beside them; its LDS array exists only
+// then, so the other passes keep their occupancy.
 template <class Items, bool kPay = false>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     Items items, uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   constexpr bool kPk = Items::kPackedItems;
   __shared__ uint32_t s_key[kTileItems];
   __shared__ int32_t s_val[kPk ? 1 : kTileItems];
-  __shared__ uint16_t s_src[kPay ? kTileItems : 1];  // block-local input index
+  __shared__ uint2 s_pay[kPay ? kTileItems : 1];
   __shared__ uint32_t s_flag[kThreads];
   uint32_t n, base, lb;
   uint4 bi = make_uint4(0u, 0u, 0u, 0u);
@@ -615,6 +616,14 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     my_wt = ex.wtot[threadIdx.x];
   }
   items.load(lb, base, wbase, n, key, val, s_key, s_val, s_flag);
+  uint2 pay[kPay ? kItems : 1];
+  if constexpr (kPay) {
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const uint32_t i = wbase + r * 64 + lane;
+      pay[r] = i < n ? ex.pin[i] : make_uint2(0u, 0u);
+    }
+  }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
@@ -679,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const uint32_t lp = s_cnt[w][(key[r] >> shift) & mask] + rank[r];
       s_key[lp] = key[r];
       if constexpr (!kPk) s_val[lp] = val[r];
-      if constexpr (kPay) s_src[lp] = (uint16_t)(i - base);
+      if constexpr (kPay) s_pay[lp] = pay[r];
     }
   }
   __syncthreads();
@@ -693,8 +702,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const int32_t vv = kPk ? (int32_t)(kk & 0xFFFFFFu) : s_val[i];
       uint2 pl = make_uint2(0u, 0u);
       if constexpr (kPay) {
-        // the payload row of this item, read from the block's input rows
-        pl = ex.pin[base + s_src[i]];
+        pl = s_pay[i];
         ex.pout[pos] = pl;
       }
       if (kout != nullptr)
