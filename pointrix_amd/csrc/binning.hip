@@ -214,11 +214,13 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(
     before = vzero<V>();
   }
   if (base >= n) return;  // block 0 of an empty scan
+  typename Store::Pre spre[kItems];
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const uint32_t idx = r * kThreads + threadIdx.x;
     const uint32_t i = base + idx;
     s_v[(idx / kItems) * kRow + idx % kItems] = i < n ? ld(i) : vzero<V>();
+    if (i < n) spre[r] = st.pre(i);
   }
   __syncthreads();
   V v[kItems];
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(
     const uint32_t idx = r * kThreads + threadIdx.x;
     const uint32_t i = base + idx;
     const uint32_t slot = (idx / kItems) * kRow + idx % kItems;
-    if (i < n) st(i, s_e[slot], s_v[slot]);
+    if (i < n) st(i, s_e[slot], s_v[slot], spre[r]);
   }
 }
 
@@ -250,9 +252,15 @@ struct LoadU32 {
   const uint32_t *p;
   __device__ uint32_t operator()(uint32_t i) const { return p[i]; }
 };
+// Store functors of the downsweep: pre(i) is requested with the scanned
+// value (the loads of a store that gathers are then in flight with the
+// scan's own), operator() stores item i with its exclusive prefix.
+struct NoPre {};
 struct StoreU32 {
+  using Pre = NoPre;
   uint32_t *p;
-  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t) const { p[i] = excl; }
+  __device__ NoPre pre(uint32_t) const { return NoPre{}; }
+  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t, NoPre) const { p[i] = excl; }
 };
 
 // --------------------------------------------------------------------------
@@ -908,14 +916,13 @@ __device__ __forceinline__ void store_block_sum(uint32_t c, uint32_t *__restrict
 }
 
 // Exact counts, 4096 splats per block (16 per thread, every load issued
-// first): per splat its rectangle's tile count and the rectangle packed as
-// (x0 | y0 << 16, w | h << 16); per block the instance total (bsum, summed
-// into M) and the number of splats with instances (vsum, the block sums of
-// the compaction scan, which then needs no reduction kernel).
+// first): per splat its tile rectangle packed as (x0 | y0 << 16, w | h << 16)
+// (the tile count is w h); per block the instance total (bsum, summed into M)
+// and the number of splats with instances (vsum, the block sums of the
+// compaction scan, which then needs no reduction kernel).
 __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict__ uv,
                                                          const int32_t *__restrict__ radius,
                                                          uint32_t n, Grid g,
-                                                         uint32_t *__restrict__ cnt,
                                                          uint2 *__restrict__ rects,
                                                          uint32_t *__restrict__ bsum,
                                                          uint32_t *__restrict__ vsum) {
@@ -942,7 +949,6 @@ __global__ __launch_bounds__(kThreads) void count_kernel(const float *__restrict
         c = w * h;
         packed = make_uint2((uint32_t)q.x0 | ((uint32_t)q.y0 << 16), w | (h << 16));
       }
-      cnt[i] = c;
       rects[i] = packed;
       sums.x += c;
       sums.y += c > 0u ? 1u : 0u;
@@ -1004,20 +1010,39 @@ struct LoadFlag {
   const uint32_t *cnt;
   __device__ uint32_t operator()(uint32_t i) const { return cnt[i] > 0 ? 1u : 0u; }
 };
+// exact binning: a splat is listed when its packed rectangle has tiles
+struct LoadRectFlag {
+  using value_type = uint32_t;
+  const uint2 *rects;
+  __device__ uint32_t operator()(uint32_t i) const {
+    const uint32_t wh = rects[i].y;
+    return (wh & 0xFFFFu) * (wh >> 16) > 0u ? 1u : 0u;
+  }
+};
 // Compaction of the splats that touch a tile: depth bits as keys, the splat
 // index as values, and (exact binning) the packed rectangle as the depth
 // sort's payload.
+struct CompactPre {
+  float depth;
+  uint2 rect;
+};
 struct StoreCompact {
+  using Pre = CompactPre;
   const float *depth;
   uint32_t *keys;
   int32_t *vals;
   const uint2 *rin;  // nullptr: no payload
   uint2 *rout;
-  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t flag) const {
+  // depth and rectangle of every splat (visible ones share lines with the
+  // others at 1.3 of 3 visible, so reading all costs about the same traffic)
+  __device__ CompactPre pre(uint32_t i) const {
+    return CompactPre{depth[i], rin != nullptr ? rin[i] : make_uint2(0u, 0u)};
+  }
+  __device__ void operator()(uint32_t i, uint32_t excl, uint32_t flag, CompactPre p) const {
     if (flag) {
-      keys[excl] = __float_as_uint(depth[i]);
+      keys[excl] = __float_as_uint(p.depth);
       vals[excl] = (int32_t)i;
-      if (rin != nullptr) rout[excl] = rin[i];
+      if (rin != nullptr) rout[excl] = p.rect;
     }
   }
 };
@@ -1828,8 +1853,10 @@ struct LoadAreaH {
   }
 };
 struct StoreOffs2 {
+  using Pre = NoPre;
   uint32_t *offs, *soffs;
-  __device__ void operator()(uint32_t i, uint2 excl, uint2) const {
+  __device__ NoPre pre(uint32_t) const { return NoPre{}; }
+  __device__ void operator()(uint32_t i, uint2 excl, uint2, NoPre) const {
     offs[i] = excl.x;
     soffs[i] = excl.y;
   }
@@ -2103,10 +2130,10 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   if (exact) {
     // the count kernel writes the compaction's block sums itself
     const uint32_t nb = nblk((uint64_t)n);
-    count_kernel<<<nb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.rects, g.bsum,
+    count_kernel<<<nb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.rects, g.bsum,
                                          g.sb.partials);
     scan_down_kernel<<<nb, kThreads, 0, s>>>(
-        LoadFlag{g.cnt},
+        LoadRectFlag{g.rects},
         StoreCompact{depth, g.sb.keyA, g.sb.valA, g.rects, reinterpret_cast<uint2 *>(g.masks)},
         nullptr, (uint32_t)n, g.sb.partials, nb,
         TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out});
