@@ -565,6 +565,8 @@ struct ScatterExtras {
   uint32_t *e_out = nullptr;
   const uint32_t *whist = nullptr;
   const uint32_t *wtot = nullptr;
+  uint32_t *e_own = nullptr;  // with e_out: e_own[k] = the item holding instance 1024 k
+  uint32_t e_own_cap = 0;
 };
 
 // kPay: the pass carries an 8-byte payload (ex.pin / ex.pout), loaded with
@@ -772,9 +774,13 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       for (int k = 0; k < kItems; ++k) {
         const uint32_t i = k * kThreads + threadIdx.x;
         if (i < cnt) {
-          const uint32_t d = (s_key[i] >> shift) & mask;
-          ex.e_out[i + (uint32_t)s_delta[d]] =
-              s_flag[d] + (uint32_t)s_val[i] + s_woff[i / (kTileItems / 4)];
+          const uint32_t kk = s_key[i];
+          const uint32_t d = (kk >> shift) & mask;
+          const uint32_t pos = i + (uint32_t)s_delta[d];
+          const uint32_t e = s_flag[d] + (uint32_t)s_val[i] + s_woff[i / (kTileItems / 4)];
+          ex.e_out[pos] = e;
+          const uint32_t kq = (e + 1023u) >> 10;  // an item holds < 1024 instances
+          if ((kq << 10) < e + (kk >> 16) && kq < ex.e_own_cap) ex.e_own[kq] = pos;
         }
       }
     }
@@ -1297,6 +1303,7 @@ struct RowStream {
   const uint32_t *ioffs;  // instance offset of each splat
   uint32_t *sc;           // scalars: [0] Nv, [1] M, [6] S in; [2..5] written
   uint32_t m;             // instance capacity
+  const uint32_t *own;    // own[k]: the splat holding segment 1024 k
 };
 
 __device__ __forceinline__ EmitWave stream_load(const RowStream &st, uint32_t nv, const Grid &,
@@ -1338,6 +1345,7 @@ struct SegStream {
   int gx, gy;
   int32_t *tile_range;          // the tiles of rows without instances get (0, 0)
   uint32_t *nblocks;            // receives the number of row blocks
+  const uint32_t *own;          // own[k]: the segment holding instance 1024 k
 };
 
 __device__ __forceinline__ EmitWave stream_load(const SegStream &st, uint32_t nv, const Grid &,
@@ -1365,9 +1373,9 @@ __device__ __forceinline__ EmitWave stream_load(const SegStream &st, uint32_t nv
 // Depth-order index of the splat holding instance q: the last s < nv with
 // offs[s] <= q (offsets strictly increase: listed splats own >= 1 instance);
 // a 64-way search, one probe per lane per step.
-__device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ offs, uint32_t nv,
-                                               uint32_t q) {
-  uint32_t lo = 0, hi = nv;
+// The same search over [lo, hi), given offs[lo] <= q.
+__device__ __forceinline__ uint32_t find_owner_in(const uint32_t *__restrict__ offs, uint32_t lo,
+                                                  uint32_t hi, uint32_t q) {
   const uint32_t lane = (uint32_t)lane_id();
   while (hi - lo > 1u) {
     const uint32_t step = (hi - lo + 63u) / 64u;
@@ -1377,6 +1385,27 @@ __device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ offs
     hi = min(lo + step, hi);
   }
   return lo;
+}
+__device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ offs, uint32_t nv,
+                                               uint32_t q) {
+  return find_owner_in(offs, 0u, nv, q);
+}
+
+// Row binning's owner tables: own[k] = the element holding item 1024 k (an
+// element holds fewer than 1024 items, so each writes at most one entry).
+// Pass A's waves start at multiples of 1024 and read their first element
+// there; pass B's waves start anywhere and search between two entries.
+__device__ __forceinline__ uint32_t stream_owner(const SplatStream &st, uint32_t nv, uint32_t q) {
+  return find_owner(st.offs, nv, q);
+}
+__device__ __forceinline__ uint32_t stream_owner(const RowStream &st, uint32_t, uint32_t q) {
+  return st.own[q >> 10];  // q is a multiple of 1024
+}
+__device__ __forceinline__ uint32_t stream_owner(const SegStream &st, uint32_t nv, uint32_t q) {
+  const uint32_t k = q >> 10;
+  const uint32_t lo = st.own[k];
+  const uint32_t hi = ((k + 1u) << 10) < st.sc[5] ? st.own[k + 1u] + 1u : nv;
+  return find_owner_in(st.offs, lo, hi, q);
 }
 
 // Instances [q0, q1) in order, from the splat s0 that holds q0, over
@@ -1604,7 +1633,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
   const uint32_t q0 = base + (uint32_t)w * kWaveItems;
   if (q0 < end) {  // uniform per wave
     const uint32_t q1 = min(q0 + kWaveItems, end);
-    const uint32_t s0 = find_owner(st.offs, nv, q0);
+    const uint32_t s0 = stream_owner(st, nv, q0);
     if (lane_id() == 0) first[lb * (kThreads / 64) + w] = s0;
     EmitWave cur = stream_load(st, nv, g, s0);
     for (uint32_t sb = s0;; sb += 64) {
@@ -1855,10 +1884,14 @@ struct LoadAreaH {
 struct StoreOffs2 {
   using Pre = NoPre;
   uint32_t *offs, *soffs;
+  uint32_t *own;     // own[k]: the splat holding segment 1024 k (k < own_cap)
+  uint32_t own_cap;
   __device__ NoPre pre(uint32_t) const { return NoPre{}; }
-  __device__ void operator()(uint32_t i, uint2 excl, uint2, NoPre) const {
+  __device__ void operator()(uint32_t i, uint2 excl, uint2 v, NoPre) const {
     offs[i] = excl.x;
     soffs[i] = excl.y;
+    const uint32_t k = (excl.y + 1023u) >> 10;  // first multiple of 1024 at or after
+    if ((k << 10) < excl.y + v.y && k < own_cap) own[k] = i;
   }
 };
 
@@ -1997,6 +2030,8 @@ constexpr int kRowGrid = 256;  // row binning: columns and rows per grid at most
 // (each row adds at most one partial block), plus one empty block whose zero
 // histogram column closes the scanned histogram
 inline uint32_t row_blocks(uint64_t m) { return nblk(m) + kRowGrid + 1; }
+// entries of pass B's owner table (one per 1024 of the m instances)
+inline uint64_t own_e_cap(uint64_t m) { return m / 1024 + 2; }
 
 // Row binning's grids: exact, one view, at most 256 x 256 tiles, splat ids
 // below 2^24 (pass B packs them with the column into one word).  sort_count
@@ -2018,13 +2053,17 @@ struct GaussWs {
   uint2 *rects;                    // exact binning: packed tile rectangles
   uint2 *rects2;                   // their ping-pong partner in the depth sort
   uint2 *partials2;                // block sums of the two-value offset scan
+  uint32_t *own1k;                 // row binning: the splat holding segment 1024 k
   SortBufs sb;
 };
+// entries of own1k: S <= 256 Nv segments (rows binning's grids have <= 256 rows)
+inline uint64_t own1k_cap(uint64_t c) { return c / 4 + 2; }
 
 inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
   return 3 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
-         3 * align256(c * 8) + align256((nblk(c) + 1) * 8) + sortbufs_bytes(c);
+         3 * align256(c * 8) + align256((nblk(c) + 1) * 8) + align256(own1k_cap(c) * 4) +
+         sortbufs_bytes(c);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -2039,6 +2078,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 8, (void **)&g.rects);
   p = carve(p, c * 8, (void **)&g.rects2);
   p = carve(p, (nblk(c) + 1) * 8, (void **)&g.partials2);
+  p = carve(p, own1k_cap(c) * 4, (void **)&g.own1k);
   carve_sortbufs(p, c, g.sb);
 }
 
@@ -2055,9 +2095,9 @@ size_t gsr_sort_ws_inst_bytes(int64_t m) {
   return sortbufs_bytes(c) + align256(64) + align256(emit_items_cap(c) * 8) +
          align256(nb * (kThreads / 64) * 4) + align256(nb * 2 * kRadix * 4) +
          align256(kRadix * 4) + 2 * align256((uint64_t)kCountTiles * 4) +
-         // row binning: pass B's first[], histogram, totals, binfo
+         // row binning: pass B's first[], histogram, totals, binfo, owner table
          align256(nbr * (kThreads / 64) * 4) + align256(nbr * kRadix * 4) +
-         align256(kRadix * 4) + align256(nbr * 16);
+         align256(kRadix * 4) + align256(nbr * 16) + align256(own_e_cap(c) * 4);
 }
 
 int gsr_set_binning_mode(int mode) {
@@ -2173,8 +2213,9 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   // offsets of each splat's instances, in depth order -> M in scalars[1]; exact
   // binning also those of its segments (rectangle rows) -> S in scalars[6]
   if (exact)
-    run_scan(LoadAreaH{reinterpret_cast<const uint2 *>(g.masks)}, StoreOffs2{g.offs, g.soffs},
-             g.scalars, cap, g.partials2, TotPair{g.scalars + 1, g.scalars + 6}, s);
+    run_scan(LoadAreaH{reinterpret_cast<const uint2 *>(g.masks)},
+             StoreOffs2{g.offs, g.soffs, g.own1k, (uint32_t)own1k_cap((uint64_t)cap)}, g.scalars,
+             cap, g.partials2, TotPair{g.scalars + 1, g.scalars + 6}, s);
   else
     run_scan(LoadU32{kc}, StoreU32{g.offs}, g.scalars, cap, g.sb.partials,
              TotU32{g.scalars + 1}, s);
@@ -2210,9 +2251,11 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     uint32_t *firstB, *histB, *totalsB;
     uint4 *binfo;
     p = carve(p, (uint64_t)nbB * (kThreads / 64) * 4, (void **)&firstB);
+    uint32_t *ownE;
     p = carve(p, (uint64_t)nbB * kRadix * 4, (void **)&histB);
     p = carve(p, (uint64_t)kRadix * 4, (void **)&totalsB);
-    carve(p, (uint64_t)nbB * 16, (void **)&binfo);
+    p = carve(p, (uint64_t)nbB * 16, (void **)&binfo);
+    carve(p, own_e_cap((uint64_t)m) * 4, (void **)&ownE);
     const uint32_t mu = (uint32_t)m;
     const int bitsA = tile_bits(gr.gy), bitsB = tile_bits(gr.gx);
     const uint2 *rects = reinterpret_cast<const uint2 *>(g.masks);
@@ -2223,7 +2266,7 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     // first instance E
     const uint32_t DA = 1u << bitsA;
     uint32_t *histA = tcnt, *totA = counts;
-    const RowStream rs{g.sb.valA, g.soffs, rects, g.offs, g.scalars, mu};
+    const RowStream rs{g.sb.valA, g.soffs, rects, g.offs, g.scalars, mu, g.own1k};
     gen_hist_kernel<RowStream><<<nb, kThreads, 0, s>>>(rs, g.scalars, (uint32_t)n, nullptr, mu,
                                                        gr, DA - 1u, histA, nb, first, nullptr);
     digit_scan_kernel<<<2 * DA, kThreads, 0, s>>>(histA, nb, totA, Cols{g.scalars + 2, mu});
@@ -2231,12 +2274,14 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     exA.e_out = ib.keyB;
     exA.whist = histA + (size_t)DA * nb;
     exA.wtot = totA + DA;
+    exA.e_own = ownE;
+    exA.e_own_cap = (uint32_t)own_e_cap((uint64_t)m);
     radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
         GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
         ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, histA, nb, totA, exA);
     // pass B: each row's instances, stably by column, straight to ids_out
-    const SegStream ss{ib.valA,  ib.keyB, ib.keyA,    totA,
-                       g.scalars, gr.gx, gr.gy, tile_range, g.scalars + 7};
+    const SegStream ss{ib.valA, ib.keyB, ib.keyA,     totA,          g.scalars,
+                       gr.gx,   gr.gy,   tile_range, g.scalars + 7, ownE};
     gen_hist_kernel<SegStream, true><<<nbB, kThreads, 0, s>>>(
         ss, g.scalars + 2, mu, g.scalars + 5, mu, gr, (1u << bitsB) - 1u, histB, nbB, firstB,
         binfo);
