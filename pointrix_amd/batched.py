@@ -193,6 +193,9 @@ class _BatchedFn(torch.autograd.Function):
         ctx.sizes = (n, V, C, K)
         ctx.op_shape = opacity.shape
         ctx.mark_non_differentiable(radius, nc)
+        # an unused T output gets no zero-filled gradient (the backward then
+        # runs its kernel variant without the background / T_final term)
+        ctx.set_materialize_grads(False)
         return out, T, radius, nc
 
     @staticmethod
