@@ -188,11 +188,14 @@ int gsr_sort_gaussian_noalloc(const float *uv, const float *depth, const int32_t
 int gsr_sort_count_tight(const float *uv, const float *depth, const int32_t *radius,
                          const float *conic, const float *opacity, int64_t n, int W, int H,
                          int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, void *stream);
-/* As gsr_sort_count_tight, and M is also copied into m_host (pinned host
- * memory) as soon as it is known -- before the depth sort, which is queued
- * behind the copy -- with `event` (a hipEvent_t, may be NULL) recorded after
- * the copy.  A host that waits on the event reads M while the device still
- * sorts, instead of draining the stream to read m_out.  No msplat counterpart
+/* As gsr_sort_count_tight, and M also reaches m_host (pinned host memory) as
+ * soon as it is known -- before the depth sort.  When m_host has a device
+ * address (hipHostGetDevicePointer) the kernel that sums M stores it there;
+ * otherwise a copy is queued.  `event` (a hipEvent_t, may be NULL) is recorded
+ * after that.  A host reads M while the device still sorts, instead of
+ * draining the stream to read m_out: by waiting on the event, or, with event
+ * NULL, by polling a value it set negative before the call (no event: no
+ * barrier in the stream ahead of the depth sort).  No msplat counterpart
  * (msplat.sort_gaussian, msplat.py:133-135, sizes its outputs internally). */
 int gsr_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
                          const float *conic, const float *opacity, int64_t n, int W, int H,
@@ -235,8 +238,8 @@ int gsr_mv_sort_count(const float *uv, const float *depth, const int32_t *radius
                       const float *conic, const float *opacity, int64_t n_per_view, int V,
                       const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,
                       int32_t *m_out, void *stream);
-/* As gsr_mv_sort_count, with M also copied into m_host (pinned) ahead of the
- * depth sort and `event` recorded after the copy (see gsr_sort_count_async). */
+/* As gsr_mv_sort_count, with M also in m_host (pinned) ahead of the depth
+ * sort and `event` recorded after it (see gsr_sort_count_async). */
 int gsr_mv_sort_count_async(const float *uv, const float *depth, const int32_t *radius,
                             const float *conic, const float *opacity, int64_t n_per_view, int V,
                             const int32_t *view_i, int tile_w, int tile_h, void *ws_gauss,

@@ -158,12 +158,11 @@ class _BatchedFn(torch.autograd.Function):
             tc, to = (ptr(conic), ptr(o)) if meta["tight"] else (None, None)
             if capacity is None:
                 # M reaches the host ahead of the depth sort (no stream drain)
-                m_host, ev = ws.sort.host_count(dev)
+                m_host = ws.sort.host_count(dev)
                 call("gsr_mv_sort_count_async", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V,
                      ptr(vi), lay.tile_w, lay.tile_h, ptr(g), ptr(m_dev),
-                     ctypes.c_void_p(m_host.data_ptr()), ctypes.c_void_p(ev.cuda_event), st)
-                ev.synchronize()
-                m = int(m_host[0])
+                     ctypes.c_void_p(m_host.data_ptr()), None, st)
+                m = ws.sort.wait_count(dev)
                 meta["count"] = m
             else:
                 call("gsr_mv_sort_count", ptr(uv), ptr(depth), ptr(radius), tc, to, n, V,

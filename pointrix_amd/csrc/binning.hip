@@ -158,11 +158,14 @@ struct TotU32 {
 };
 // the compaction: the total (visible splats) and the plain sum of aux[0, naux)
 // (the count blocks' instance totals: M)
+// (the count also goes straight to host_out, a device-visible pinned host
+// word, when given: no copy is queued behind the kernel)
 struct TotCompact {
   uint32_t *out;
   const uint32_t *aux;
   uint32_t naux;
   uint32_t *aux_out;
+  uint32_t *host_out = nullptr;
   __device__ void operator()(uint32_t t) const {
     uint32_t a = 0;
 #pragma unroll 4
@@ -172,6 +175,7 @@ struct TotCompact {
     if (threadIdx.x == 0) {
       *out = t;
       *aux_out = sum;
+      if (host_out != nullptr) *host_out = sum;
     }
   }
 };
@@ -2110,17 +2114,32 @@ int gsr_set_binning_mode(int mode) {
 
 namespace {
 
-// M to the host copy as soon as it is known (m_host, pinned; event recorded
-// after the copy), ahead of the depth sort that is queued behind it.
-int copy_count_to_host(const int32_t *m_out, int32_t *m_host, void *event, hipStream_t s) {
+// M to the host as soon as it is known (m_host, pinned; event recorded after
+// it), ahead of the depth sort that is queued behind it.  written: the kernel
+// that sums M already stored it through the host word's device address.
+int copy_count_to_host(const int32_t *m_out, int32_t *m_host, void *event, hipStream_t s,
+                       bool written = false) {
   if (m_host == nullptr) return GSR_OK;
-  GSR_REQUIRE(hipMemcpyAsync(m_host, m_out, sizeof(int32_t), hipMemcpyDeviceToHost, s) ==
-                  hipSuccess,
-              "sort_count: instance count copy failed");
+  if (!written)
+    GSR_REQUIRE(hipMemcpyAsync(m_host, m_out, sizeof(int32_t), hipMemcpyDeviceToHost, s) ==
+                    hipSuccess,
+                "sort_count: instance count copy failed");
   if (event != nullptr)
     GSR_REQUIRE(hipEventRecord((hipEvent_t)event, s) == hipSuccess,
                 "sort_count: event record failed");
   return GSR_OK;
+}
+
+// Device address of a pinned host word (nullptr if it has none: the count is
+// then copied).
+uint32_t *host_word_on_device(int32_t *m_host) {
+  if (m_host == nullptr) return nullptr;
+  void *d = nullptr;
+  if (hipHostGetDevicePointer(&d, m_host, 0) != hipSuccess) {
+    (void)hipGetLastError();  // clear it: the copy path is taken instead
+    return nullptr;
+  }
+  return static_cast<uint32_t *>(d);
 }
 
 // Scratch of gsr_sort_gaussian_noalloc: one file-scope device array, which
@@ -2165,8 +2184,9 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     return rc != GSR_OK ? rc : gsr_launch_status("sort_count");
   }
   // compact visible splats with their depth bits as keys -> Nv in scalars[0];
-  // the same launch sums the count blocks' totals -> M in m_out
+  // the same launch sums the count blocks' totals -> M in m_out (and in m_host)
   const bool exact = gr.conic == nullptr;
+  uint32_t *m_host_dev = host_word_on_device(m_host);
   if (exact) {
     // the count kernel writes the compaction's block sums itself
     const uint32_t nb = nblk((uint64_t)n);
@@ -2176,17 +2196,17 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
         LoadRectFlag{g.rects},
         StoreCompact{depth, g.sb.keyA, g.sb.valA, g.rects, reinterpret_cast<uint2 *>(g.masks)},
         nullptr, (uint32_t)n, g.sb.partials, nb,
-        TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out});
+        TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out, m_host_dev});
   } else {
     const uint32_t ncb = (uint32_t)gsr_blocks(n, kThreads);
     count_tight_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.masks,
                                                 g.bsum);
     run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA, nullptr, nullptr},
-             nullptr, (uint32_t)n, g.sb.partials, TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out},
-             s);
+             nullptr, (uint32_t)n, g.sb.partials,
+             TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out, m_host_dev}, s);
   }
   {
-    const int rc = copy_count_to_host(m_out, m_host, event, s);
+    const int rc = copy_count_to_host(m_out, m_host, event, s, m_host_dev != nullptr);
     if (rc != GSR_OK) return rc;
   }
   // depth sort of the visible splats (4 passes over 32 bits); for tight

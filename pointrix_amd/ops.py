@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import ctypes
 
+import time
+
 import torch
 
 from . import _lib
@@ -256,15 +258,33 @@ class SortWorkspace:
         return self.m_dev
 
     def host_count(self, device):
-        """(pinned host int32[1], event) the instance count is copied into
-        ahead of the depth sort; the event is created here (a first record)
-        so its handle exists before the library records it."""
+        """Pinned host int32[1] the instance count reaches ahead of the depth
+        sort (the library's count kernel stores it there).  It is set to -1
+        here; ``wait_count`` polls it."""
         if getattr(self, "m_host", None) is None or self.m_host_dev != device:
             self.m_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
-            self.m_event = torch.cuda.Event()
-            self.m_event.record(torch.cuda.current_stream(device))
             self.m_host_dev = device
-        return self.m_host, self.m_event
+        self.m_host[0] = -1
+        return self.m_host
+
+    def wait_count(self, device, timeout_s=30.0):
+        """The count of the sort just queued, read by polling the pinned word
+        (no event, so no barrier in the stream ahead of the depth sort).  If
+        it has not arrived by the deadline the device is synchronised -- a
+        failed launch then raises there -- and the word is read."""
+        m_host = self.m_host
+        m = int(m_host[0])
+        if m >= 0:
+            return m
+        deadline = time.perf_counter() + timeout_s
+        while m < 0 and time.perf_counter() < deadline:
+            m = int(m_host[0])
+        if m < 0:
+            torch.cuda.synchronize(device)
+            m = int(m_host[0])
+            if m < 0:
+                raise _lib.GsrError("sort_count: the instance count never reached the host")
+        return m
 
     def instances(self) -> int:
         """Device instance count of the last sort (host read)."""
@@ -328,13 +348,12 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
     if ws.capacity is None:
         # M reaches the host ahead of the depth sort: waiting for it does not
         # drain the stream, so the sort keeps the device busy meanwhile
-        m_host, ev = ws.host_count(dev)
+        m_host = ws.host_count(dev)
         with _span("sort_count", dev):
             call("gsr_sort_count_async", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H),
-                 tile_w, tile_h, ptr(g), ptr(m_dev), ctypes.c_void_p(m_host.data_ptr()),
-                 ctypes.c_void_p(ev.cuda_event), st)
-        ev.synchronize()
-        m = int(m_host[0])
+                 tile_w, tile_h, ptr(g), ptr(m_dev), ctypes.c_void_p(m_host.data_ptr()), None,
+                 st)
+        m = ws.wait_count(dev)
         global last_instance_count
         last_instance_count = m
     else:
