@@ -285,12 +285,14 @@ struct TileCount {
   uint32_t *over;  // [T], zero on entry
 };
 
+template <int kIt = kItems>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys,
                                                               const uint32_t *n_ptr,
                                                               uint32_t cap, int shift,
                                                               uint32_t mask,
                                                               uint32_t *__restrict__ hist,
                                                               uint32_t nb, TileCount tc) {
+  constexpr uint32_t kBlk = kThreads * kIt;
   __shared__ uint32_t s_h[kRadix];
   __shared__ uint32_t s_t[2][kRadix];
   s_h[threadIdx.x] = 0;
@@ -298,8 +300,8 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   s_t[1][threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n = count_of(n_ptr, cap);
-  const uint32_t lb = logical_block(blocks_of(n));
-  const uint32_t base = lb * kTileItems;
+  const uint32_t lb = logical_block((n + kBlk - 1) / kBlk);
+  const uint32_t base = lb * kBlk;
   if (base >= n) return;  // no items: no column (the digit scan covers the others)
   const bool tiles = tc.tcnt != nullptr;
   const uint32_t lmask = (1u << shift) - 1u;
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   // a block of one low digit (nearly all of them) has its slot-0 counts in
   // s_h already; only blocks that span several count per slot
   const bool split =
-      tiles && base < n && (keys[min(base + kTileItems, n) - 1] & lmask) != lf;
+      tiles && base < n && (keys[min(base + kBlk, n) - 1] & lmask) != lf;
   auto count = [&](uint32_t k) {
     const uint32_t d = (k >> shift) & mask;
     atomicAdd(&s_h[d], 1u);
@@ -319,15 +321,15 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
         atomicAdd(&tc.over[k], 1u);
     }
   };
-  if (base + kTileItems <= n) {
+  if (base + kBlk <= n) {
     // full block: 16-byte loads, all in flight before the LDS counts
     // (keys and the block base are 16-byte aligned)
-    uint4 v[kItems / 4];
+    uint4 v[kIt / 4];
 #pragma unroll
-    for (int k = 0; k < kItems / 4; ++k)
+    for (int k = 0; k < kIt / 4; ++k)
       v[k] = reinterpret_cast<const uint4 *>(keys + base)[k * kThreads + threadIdx.x];
 #pragma unroll
-    for (int k = 0; k < kItems / 4; ++k) {
+    for (int k = 0; k < kIt / 4; ++k) {
       count(v[k].x);
       count(v[k].y);
       count(v[k].z);
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
     }
   } else if (base < n) {
 #pragma unroll 4
-    for (int k = 0; k < kItems; ++k) {
+    for (int k = 0; k < kIt; ++k) {
       const uint32_t i = base + k * kThreads + threadIdx.x;
       if (i < n) count(keys[i]);
     }
@@ -526,12 +528,13 @@ struct ListItems {
   const int32_t *vin;
   // rows wbase + 64 r + lane of this wave (every load issued before the
   // ranking, whose wave barriers would otherwise expose one latency per row)
+  template <int kIt>
   __device__ __forceinline__ void load(uint32_t, uint32_t, uint32_t wbase, uint32_t n,
-                                       uint32_t (&key)[kItems], int32_t (&val)[kItems],
+                                       uint32_t (&key)[kIt], int32_t (&val)[kIt],
                                        uint32_t *, int32_t *, uint32_t *) const {
     const int lane = lane_id();
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
+    for (int r = 0; r < kIt; ++r) {
       const uint32_t i = wbase + r * 64 + lane;
       const bool valid = i < n;
       key[r] = valid ? kin[i] : 0u;
@@ -576,20 +579,21 @@ struct ScatterExtras {
 // kPay: the pass carries an 8-byte payload (ex.pin / ex.pout), loaded with
 // the items and staged through LDS beside them; its LDS array exists only
 // then, so the other passes keep their occupancy.
-template <class Items, bool kPay = false>
+template <class Items, bool kPay = false, int kIt = kItems>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     Items items, uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
     const uint32_t *__restrict__ totals, ScatterExtras ex) {
   static_assert(kRadix == kThreads, "one digit per thread");
+  constexpr uint32_t kBlk = kThreads * kIt;
   __shared__ uint32_t s_cnt[kThreads / 64][kRadix];  // per-wave counts, then LDS offsets
   __shared__ int32_t s_delta[kRadix];                // global start - LDS start per digit
   // packed items (Items::kPackedItems): the key word carries the value in its
   // low 24 bits, and no value array is staged
   constexpr bool kPk = Items::kPackedItems;
-  __shared__ uint32_t s_key[kTileItems];
-  __shared__ int32_t s_val[kPk ? 1 : kTileItems];
-  __shared__ uint2 s_pay[kPay ? kTileItems : 1];
+  __shared__ uint32_t s_key[kBlk];
+  __shared__ int32_t s_val[kPk ? 1 : kBlk];
+  __shared__ uint2 s_pay[kPay ? kBlk : 1];
   __shared__ uint32_t s_flag[kThreads];
   uint32_t n, base, lb;
   uint4 bi = make_uint4(0u, 0u, 0u, 0u);
@@ -600,8 +604,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     n = bi.y;
   } else {
     n = count_of(n_ptr, cap);
-    lb = logical_block(blocks_of(n));
-    base = lb * kTileItems;
+    lb = logical_block((n + kBlk - 1) / kBlk);
+    base = lb * kBlk;
   }
   if (base >= n) return;  // uniform over the block
   const uint32_t mask = (1u << bits) - 1u;
@@ -609,10 +613,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
 #pragma unroll
   for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][threadIdx.x] = 0;
   const uint64_t lt = lanemask_lt();
-  uint32_t key[kItems];
-  int32_t val[kItems];
-  uint32_t rank[kItems];
-  const uint32_t wbase = base + w * (kTileItems / (kThreads / 64));
+  uint32_t key[kIt];
+  int32_t val[kIt];
+  uint32_t rank[kIt];
+  const uint32_t wbase = base + w * (kBlk / (kThreads / 64));
   // this thread's digit total and block offset, requested with the items
   const bool my_digit = (int)threadIdx.x < (1 << bits);
   const uint32_t my_total = my_digit ? totals[threadIdx.x] : 0u;
@@ -629,18 +633,18 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     my_w = ex.whist[threadIdx.x * nb + lb];
     my_wt = ex.wtot[threadIdx.x];
   }
-  items.load(lb, base, wbase, n, key, val, s_key, s_val, s_flag);
-  uint2 pay[kPay ? kItems : 1];
+  items.template load<kIt>(lb, base, wbase, n, key, val, s_key, s_val, s_flag);
+  uint2 pay[kPay ? kIt : 1];
   if constexpr (kPay) {
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
+    for (int r = 0; r < kIt; ++r) {
       const uint32_t i = wbase + r * 64 + lane;
       pay[r] = i < n ? ex.pin[i] : make_uint2(0u, 0u);
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
+  for (int r = 0; r < kIt; ++r) {
     const uint32_t i = wbase + r * 64 + lane;
     const bool valid = i < n;
     const uint32_t d = (key[r] >> shift) & mask;
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       // then d in the row's earlier blocks
       const uint32_t blk0 = bi.z, ty = bi.w & 0xFFu;
       const uint32_t c = row_p1 - row_p0;
-      const uint32_t e0 = base - (lb - blk0) * kTileItems;
+      const uint32_t e0 = base - (lb - blk0) * kBlk;
       const uint32_t tstart = e0 + block_exclusive_scan(c, total);
       s_delta[d] = (int32_t)(tstart + (my_hist - row_p0) - lstart);
       if (lb == blk0 && d < ex.gx) {
@@ -696,7 +700,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
+  for (int r = 0; r < kIt; ++r) {
     const uint32_t i = wbase + r * 64 + lane;
     if (i < n) {
       const uint32_t lp = s_cnt[w][(key[r] >> shift) & mask] + rank[r];
@@ -706,9 +710,9 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     }
   }
   __syncthreads();
-  const uint32_t cnt = min((uint32_t)kTileItems, n - base);
+  const uint32_t cnt = min((uint32_t)kBlk, n - base);
 #pragma unroll 4
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < kIt; ++k) {
     const uint32_t i = k * kThreads + threadIdx.x;
     if (i < cnt) {
       const uint32_t kk = s_key[i];
@@ -737,8 +741,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       const int wv = threadIdx.x >> 6, ln = lane_id();
       uint32_t carry = 0;
 #pragma unroll 4
-      for (int r = 0; r < kItems; ++r) {
-        const uint32_t idx = (uint32_t)wv * (kTileItems / 4) + r * 64 + ln;
+      for (int r = 0; r < kIt; ++r) {
+        const uint32_t idx = (uint32_t)wv * (kBlk / 4) + r * 64 + ln;
         const uint32_t wd = idx < cnt ? s_key[idx] >> 16 : 0u;
         uint32_t inc = wd;
 #pragma unroll
@@ -770,18 +774,18 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
         const uint32_t dstart = block_exclusive_scan(my_wt, tot);
         const uint32_t ls = s_cnt[0][d];
         const uint32_t gls =
-            ls < cnt ? (uint32_t)s_val[ls] + s_woff[ls / (kTileItems / 4)] : 0u;
+            ls < cnt ? (uint32_t)s_val[ls] + s_woff[ls / (kBlk / 4)] : 0u;
         s_flag[d] = dstart + my_w - gls;
       }
       __syncthreads();
 #pragma unroll 4
-      for (int k = 0; k < kItems; ++k) {
+      for (int k = 0; k < kIt; ++k) {
         const uint32_t i = k * kThreads + threadIdx.x;
         if (i < cnt) {
           const uint32_t kk = s_key[i];
           const uint32_t d = (kk >> shift) & mask;
           const uint32_t pos = i + (uint32_t)s_delta[d];
-          const uint32_t e = s_flag[d] + (uint32_t)s_val[i] + s_woff[i / (kTileItems / 4)];
+          const uint32_t e = s_flag[d] + (uint32_t)s_val[i] + s_woff[i / (kBlk / 4)];
           ex.e_out[pos] = e;
           const uint32_t kq = (e + 1023u) >> 10;  // an item holds < 1024 instances
           if ((kq << 10) < e + (kk >> 16) && kq < ex.e_own_cap) ex.e_own[kq] = pos;
@@ -1725,9 +1729,11 @@ struct GenItems {
   Grid g;
   const uint32_t *first;
   const uint32_t *fix;
+  template <int kIt>
   __device__ __forceinline__ void load(uint32_t block, uint32_t base, uint32_t wbase, uint32_t n,
-                                       uint32_t (&key)[kItems], int32_t (&val)[kItems],
+                                       uint32_t (&key)[kIt], int32_t (&val)[kIt],
                                        uint32_t *s_key, int32_t *s_val, uint32_t *s_flag) const {
+    static_assert(kIt == kItems, "generated blocks hold 4096 items");
     const int w = threadIdx.x >> 6, lane = lane_id();
     constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
     if (wbase < n) {  // uniform per wave
@@ -1904,6 +1910,18 @@ struct StoreOffs2 {
 // --------------------------------------------------------------------------
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 inline uint32_t nblk(uint64_t n) { return (uint32_t)((n + kTileItems - 1) / kTileItems); }
+// radix blocks of kThreads * it items
+inline uint32_t nblk_it(uint64_t n, int it) {
+  const uint64_t b = (uint64_t)kThreads * it;
+  return (uint32_t)((n + b - 1) / b);
+}
+// The depth sort (a few million keys at most) runs blocks of 1024 keys: four
+// times the blocks of the tile passes' 4096, so ~5 per CU hide each other's
+// load, ranking and scatter latencies (one 4096-key block per CU leaves them
+// exposed).  A digit's output run per block is then short, but neighbouring
+// blocks run on one XCD (logical_block) and their partial lines merge in its
+// L2.  (Measured before the XCD-aware order, 1024-key blocks were slower.)
+constexpr int kDepthItems = 4;
 
 // Workspace carve-up for a capacity of `cap` items of 32-bit keys + values.
 struct SortBufs {
@@ -1914,9 +1932,12 @@ struct SortBufs {
   uint32_t *partials;  // data scan partials
 };
 
-inline size_t sortbufs_bytes(uint64_t cap) {
+// hist holds the histograms of radix blocks of kThreads * hist_it items;
+// partials the block sums of 4096-item scan blocks
+inline size_t sortbufs_bytes(uint64_t cap, int hist_it = kItems) {
   const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
-  const uint64_t hn = (uint64_t)kRadix * nb;
+  const uint64_t nh = nblk_it(cap, hist_it) ? nblk_it(cap, hist_it) : 1;
+  const uint64_t hn = (uint64_t)kRadix * nh;
   return 4 * align256(cap * 4) + align256(hn * 4) + align256(kRadix * 4) +
          align256((nb + 1) * 4);
 }
@@ -1926,9 +1947,10 @@ inline char *carve(char *p, size_t bytes, void **out) {
   return p + align256(bytes);
 }
 
-inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b) {
+inline char *carve_sortbufs(char *p, uint64_t cap, SortBufs &b, int hist_it = kItems) {
   const uint64_t nb = nblk(cap) ? nblk(cap) : 1;
-  const uint64_t hn = (uint64_t)kRadix * nb;
+  const uint64_t nh = nblk_it(cap, hist_it) ? nblk_it(cap, hist_it) : 1;
+  const uint64_t hn = (uint64_t)kRadix * nh;
   p = carve(p, cap * 4, (void **)&b.keyA);
   p = carve(p, cap * 4, (void **)&b.keyB);
   p = carve(p, cap * 4, (void **)&b.valA);
@@ -1968,10 +1990,11 @@ struct RadixOpts {
   uint32_t *totals0 = nullptr;  // the first pass's digit totals, kept apart
 };
 
+template <int kIt = kItems>
 void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_alt,
                const uint32_t *n_ptr, uint32_t cap, int nbits, SortBufs &b, hipStream_t s,
                RadixOpts o = RadixOpts{}) {
-  const uint32_t nb = nblk(cap) ? nblk(cap) : 1;
+  const uint32_t nb = nblk_it(cap, kIt) ? nblk_it(cap, kIt) : 1;
   const int passes = radix_passes(nbits);
   const int per = passes ? (nbits + passes - 1) / passes : 0;
   for (int p = o.p0; p < passes; ++p) {
@@ -1980,9 +2003,10 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     const uint32_t mask = (1u << bits) - 1u;
     const bool last = p == passes - 1;
     uint32_t *totals = (p == 0 && o.totals0 != nullptr) ? o.totals0 : b.totals;
-    radix_hist_kernel<<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb,
-                                              last ? o.final_tc : TileCount{});
-    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals, Cols{n_ptr, cap});
+    radix_hist_kernel<kIt><<<nb, kThreads, 0, s>>>(k_cur, n_ptr, cap, shift, mask, b.hist, nb,
+                                                   last ? o.final_tc : TileCount{});
+    digit_scan_kernel<<<1u << bits, kThreads, 0, s>>>(b.hist, nb, totals,
+                                                      Cols{n_ptr, cap, kThreads * kIt, 0});
     ScatterExtras ex;
     ex.pin = o.pay_cur;
     ex.pout = o.pay_alt;
@@ -1993,11 +2017,11 @@ void run_radix(uint32_t *&k_cur, int32_t *&v_cur, uint32_t *k_alt, int32_t *v_al
     }
     const bool keys_out = !last || o.final_keys || o.final_gather != nullptr || o.final_area;
     if (ex.pin != nullptr)
-      radix_scatter_kernel<ListItems, true><<<nb, kThreads, 0, s>>>(
+      radix_scatter_kernel<ListItems, true, kIt><<<nb, kThreads, 0, s>>>(
           ListItems{k_cur, v_cur}, keys_out ? k_alt : nullptr, v_alt, n_ptr, cap, shift, bits,
           b.hist, nb, totals, ex);
     else
-      radix_scatter_kernel<ListItems, false><<<nb, kThreads, 0, s>>>(
+      radix_scatter_kernel<ListItems, false, kIt><<<nb, kThreads, 0, s>>>(
           ListItems{k_cur, v_cur}, keys_out ? k_alt : nullptr, v_alt, n_ptr, cap, shift, bits,
           b.hist, nb, totals, ex);
     uint32_t *kt = k_cur;
@@ -2067,7 +2091,7 @@ inline size_t gauss_ws_bytes(uint64_t n) {
   const uint64_t c = n ? n : 1;
   return 3 * align256(c * 4) + align256(64) + align256((c / kThreads + 1) * 4) +
          3 * align256(c * 8) + align256((nblk(c) + 1) * 8) + align256(own1k_cap(c) * 4) +
-         sortbufs_bytes(c);
+         sortbufs_bytes(c, kDepthItems);
 }
 
 inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
@@ -2083,7 +2107,7 @@ inline void carve_gauss(void *ws, uint64_t n, GaussWs &g) {
   p = carve(p, c * 8, (void **)&g.rects2);
   p = carve(p, (nblk(c) + 1) * 8, (void **)&g.partials2);
   p = carve(p, own1k_cap(c) * 4, (void **)&g.own1k);
-  carve_sortbufs(p, c, g.sb);
+  carve_sortbufs(p, c, g.sb, kDepthItems);
 }
 
 }  // namespace
@@ -2229,7 +2253,7 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
     o.final_gather = g.cnt;
     o.final_rank = reinterpret_cast<uint32_t *>(g.rects);
   }
-  run_radix(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, o);
+  run_radix<kDepthItems>(kc, vc, g.sb.keyB, g.sb.valB, g.scalars, cap, 32, g.sb, s, o);
   // offsets of each splat's instances, in depth order -> M in scalars[1]; exact
   // binning also those of its segments (rectangle rows) -> S in scalars[6]
   if (exact)
