@@ -648,22 +648,30 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const uint32_t i = wbase + r * 64 + lane;
     const bool valid = i < n;
     const uint32_t d = (key[r] >> shift) & mask;
-    uint64_t m = __ballot(valid);
+    // m = the valid lanes whose digit equals this lane's: one ballot per
+    // digit bit, all kRadixBits of them with no branch (the bits at and above
+    // `bits` are zero in every digit, so their ballots keep m as it is)
+    // (per bit: v_bfe_i32, v_cmp into the ballot, two xor and the and; the
+    // empty asm keeps the compare on clr instead of a shift of the digit)
+    const uint64_t vm = __ballot(valid);
+    uint32_t mlo = (uint32_t)vm, mhi = (uint32_t)(vm >> 32);
+    const uint32_t nd = ~d;
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
-      if (b < bits) {  // uniform
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        m &= bit ? bal : ~bal;
-      }
+      uint32_t clr = (uint32_t)__builtin_amdgcn_sbfe((int)nd, b, 1);  // 0: bit b set, ~0: clear
+      asm("" : "+v"(clr));
+      const uint64_t bal = __ballot(clr == 0u);
+      mlo &= (uint32_t)bal ^ clr;
+      mhi &= (uint32_t)(bal >> 32) ^ clr;
     }
-    uint32_t prior = 0;
-    if (valid) prior = s_cnt[w][d];
+    const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+    const uint32_t before = (uint32_t)__popcll(m & lt);
+    const uint32_t prior = s_cnt[w][d];  // d is in range on every lane
     __builtin_amdgcn_wave_barrier();
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    if (valid && lane == leader) s_cnt[w][d] = prior + (uint32_t)__popcll(m);
+    // the group's last lane stores the digit's count after this row
+    if (valid && (m >> lane) == 1ull) s_cnt[w][d] = prior + before + 1u;
     __builtin_amdgcn_wave_barrier();
-    rank[r] = prior + (uint32_t)__popcll(m & lt);
+    rank[r] = prior + before;
   }
   __syncthreads();
   {
