@@ -55,16 +55,30 @@ __device__ __forceinline__ uint32_t blocks_of(uint32_t n) {
   return (n + kTileItems - 1) / kTileItems;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave in registers (DPP, no LDS
+// round trip; every lane of the wave active): row_shr 1, 2, 4, 8 scan each
+// row of 16 lanes (a lane whose source is outside its row adds the 0 of
+// `old`), row_bcast:15 on rows 1 and 3 adds lane 15's and lane 47's sums,
+// row_bcast:31 on rows 2 and 3 adds lane 31's.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)x;
+}
+__device__ __forceinline__ uint32_t wave_last(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
   __shared__ uint32_t s_w[kThreads / 64];
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  uint32_t inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += t;
-  }
+  const uint32_t inc = wave_inclusive_scan(v);
   if (lane == 63) s_w[w] = inc;
   __syncthreads();
   uint32_t base = 0;
@@ -91,9 +105,9 @@ __device__ __forceinline__ uint32_t vsub(uint32_t a, uint32_t b) { return a - b;
 __device__ __forceinline__ uint2 vsub(uint2 a, uint2 b) {
   return make_uint2(a.x - b.x, a.y - b.y);
 }
-__device__ __forceinline__ uint32_t vshfl_up(uint32_t x, int d) { return __shfl_up(x, d, 64); }
-__device__ __forceinline__ uint2 vshfl_up(uint2 x, int d) {
-  return make_uint2(__shfl_up(x.x, d, 64), __shfl_up(x.y, d, 64));
+__device__ __forceinline__ uint32_t vscan(uint32_t x) { return wave_inclusive_scan(x); }
+__device__ __forceinline__ uint2 vscan(uint2 x) {
+  return make_uint2(wave_inclusive_scan(x.x), wave_inclusive_scan(x.y));
 }
 template <class V>
 __device__ __forceinline__ V vzero() {
@@ -105,12 +119,7 @@ template <class V>
 __device__ __forceinline__ V block_exclusive_scan_v(V v, V &total) {
   __shared__ V s_wv[kThreads / 64];
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  V inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const V t = vshfl_up(inc, d);
-    if (lane >= d) inc = vadd(inc, t);
-  }
+  const V inc = vscan(v);
   if (lane == 63) s_wv[w] = inc;
   __syncthreads();
   V base = vzero<V>();
@@ -422,12 +431,7 @@ __global__ __launch_bounds__(1024) void ranges_from_counts_kernel(
       v[j] = t0 + j < T ? counts[t0 + j] : 0u;
       sum += v[j];
     }
-    uint32_t inc = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t u = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += u;
-    }
+    const uint32_t inc = wave_inclusive_scan(sum);
     if (lane == 63) s_w[w] = inc;
     __syncthreads();
     uint32_t run = s_carry + inc - sum, tot = 0;
@@ -752,14 +756,9 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       for (int r = 0; r < kIt; ++r) {
         const uint32_t idx = (uint32_t)wv * (kBlk / 4) + r * 64 + ln;
         const uint32_t wd = idx < cnt ? s_key[idx] >> 16 : 0u;
-        uint32_t inc = wd;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const uint32_t t = __shfl_up(inc, d, 64);
-          if (ln >= d) inc += t;
-        }
+        const uint32_t inc = wave_inclusive_scan(wd);
         if (idx < cnt) s_val[idx] = (int32_t)(carry + inc - wd);
-        carry += (uint32_t)__shfl((int)inc, 63, 64);
+        carry += wave_last(inc);
       }
       if (ln == 0) s_woff[wv] = carry;
       __syncthreads();
@@ -913,17 +912,12 @@ __device__ __forceinline__ bool owner_test(const TLane &L, int o, int tx, int ty
   return tile_reachable(t, (float)px0, (float)px1, (float)py0, (float)py1);
 }
 
-// exclusive prefix of v over the wave; *total = the wave's sum
+// exclusive prefix of v over the wave (all lanes active); *total = the
+// wave's sum
 __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
-  const int lane = lane_id();
-  int inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += t;
-  }
-  total = __shfl(inc, 63, 64);
-  return inc - v;
+  const uint32_t inc = wave_inclusive_scan((uint32_t)v);
+  total = (int)wave_last(inc);
+  return (int)inc - v;
 }
 
 // Per splat: tile count of its rectangle, and the rectangle packed as
