@@ -55,6 +55,14 @@ __device__ __forceinline__ uint32_t blocks_of(uint32_t n) {
   return (n + kTileItems - 1) / kTileItems;
 }
 
+// The wave's index in its block as a scalar: the compiler cannot tell that
+// threadIdx.x >> 6 is the same on every lane of a wave, and would otherwise
+// keep it and every address and loop bound derived from it in vector
+// registers (loop control then runs on the VALU with exec-mask updates).
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave in registers (DPP, no LDS
 // round trip; every lane of the wave active): row_shr 1, 2, 4, 8 scan each
 // row of 16 lanes (a lane whose source is outside its row adds the 0 of
@@ -77,7 +85,7 @@ __device__ __forceinline__ uint32_t wave_last(uint32_t x) {
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
   __shared__ uint32_t s_w[kThreads / 64];
-  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int lane = lane_id(), w = wave_index();
   const uint32_t inc = wave_inclusive_scan(v);
   if (lane == 63) s_w[w] = inc;
   __syncthreads();
@@ -118,7 +126,7 @@ __device__ __forceinline__ V vzero() {
 template <class V>
 __device__ __forceinline__ V block_exclusive_scan_v(V v, V &total) {
   __shared__ V s_wv[kThreads / 64];
-  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int lane = lane_id(), w = wave_index();
   const V inc = vscan(v);
   if (lane == 63) s_wv[w] = inc;
   __syncthreads();
@@ -420,7 +428,7 @@ __global__ __launch_bounds__(1024) void ranges_from_counts_kernel(
     const uint32_t *__restrict__ counts, int T, int32_t *__restrict__ tile_range) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
-  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int lane = lane_id(), w = wave_index();
   if (threadIdx.x == 0) s_carry = 0;
   __syncthreads();
   for (int r0 = 0; r0 < T; r0 += 4096) {
@@ -583,7 +591,9 @@ struct ScatterExtras {
 // kPay: the pass carries an 8-byte payload (ex.pin / ex.pout), loaded with
 // the items and staged through LDS beside them; its LDS array exists only
 // then, so the other passes keep their occupancy.
-template <class Items, bool kPay = false, int kIt = kItems>
+// kBits: the digit bits the ranking ballots (bits >= kBits are zero in every
+// digit of the pass; their ballots would keep the match masks as they are).
+template <class Items, bool kPay = false, int kIt = kItems, int kBits = kRadixBits>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     Items items, uint32_t *__restrict__ kout, int32_t *__restrict__ vout, const uint32_t *n_ptr,
     uint32_t cap, int shift, int bits, const uint32_t *__restrict__ hist_scanned, uint32_t nb,
@@ -613,7 +623,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
   }
   if (base >= n) return;  // uniform over the block
   const uint32_t mask = (1u << bits) - 1u;
-  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int w = wave_index(), lane = lane_id();
 #pragma unroll
   for (int k = 0; k < kThreads / 64; ++k) s_cnt[k][threadIdx.x] = 0;
   const uint64_t lt = lanemask_lt();
@@ -653,7 +663,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const bool valid = i < n;
     const uint32_t d = (key[r] >> shift) & mask;
     // m = the valid lanes whose digit equals this lane's: one ballot per
-    // digit bit, all kRadixBits of them with no branch (the bits at and above
+    // digit bit, all kBits of them with no branch (the bits at and above
     // `bits` are zero in every digit, so their ballots keep m as it is)
     // (per bit: v_bfe_i32, v_cmp into the ballot, two xor and the and; the
     // empty asm keeps the compare on clr instead of a shift of the digit)
@@ -661,7 +671,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     uint32_t mlo = (uint32_t)vm, mhi = (uint32_t)(vm >> 32);
     const uint32_t nd = ~d;
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
+    for (int b = 0; b < kBits; ++b) {
       uint32_t clr = (uint32_t)__builtin_amdgcn_sbfe((int)nd, b, 1);  // 0: bit b set, ~0: clear
       asm("" : "+v"(clr));
       const uint64_t bal = __ballot(clr == 0u);
@@ -750,7 +760,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
       // kept in s_val (read above, free now); s_woff = the waves' offsets
       __shared__ uint32_t s_woff[kThreads / 64];
       __syncthreads();
-      const int wv = threadIdx.x >> 6, ln = lane_id();
+      const int wv = wave_index(), ln = lane_id();
       uint32_t carry = 0;
 #pragma unroll 4
       for (int r = 0; r < kIt; ++r) {
@@ -1636,7 +1646,7 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
     base = lb * kTileItems;
   }
   const uint32_t nv = count_of(nv_ptr, cap_n);
-  const int w = threadIdx.x >> 6;
+  const int w = wave_index();
   const uint32_t D = mask + 1u;
   const uint32_t dbits = (uint32_t)__popc(mask);
   constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
@@ -1736,7 +1746,7 @@ struct GenItems {
                                        uint32_t (&key)[kIt], int32_t (&val)[kIt],
                                        uint32_t *s_key, int32_t *s_val, uint32_t *s_flag) const {
     static_assert(kIt == kItems, "generated blocks hold 4096 items");
-    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int w = wave_index(), lane = lane_id();
     constexpr uint32_t kWaveItems = kTileItems / (kThreads / 64);
     if (wbase < n) {  // uniform per wave
       GenStage<kFix, kPacked> f{s_key, s_val, base, 0xFFFFFFFFu, 0u};
@@ -2268,6 +2278,24 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
   return gsr_launch_status("sort_count");
 }
 
+// A generated pass's scatter with its ranking cut to the digit width (6, 7
+// or 8 bits: the tile rows and columns of a grid up to 64, 128 or 256 wide).
+template <class Items>
+void gen_scatter(int bits, uint32_t nblocks, hipStream_t s, const Items &it, uint32_t *kout,
+                 int32_t *vout, const uint32_t *n_ptr, uint32_t cap, int shift,
+                 const uint32_t *hist, uint32_t nb, const uint32_t *totals,
+                 const ScatterExtras &ex) {
+  if (bits <= 6)
+    radix_scatter_kernel<Items, false, kItems, 6><<<nblocks, kThreads, 0, s>>>(
+        it, kout, vout, n_ptr, cap, shift, bits, hist, nb, totals, ex);
+  else if (bits == 7)
+    radix_scatter_kernel<Items, false, kItems, 7><<<nblocks, kThreads, 0, s>>>(
+        it, kout, vout, n_ptr, cap, shift, bits, hist, nb, totals, ex);
+  else
+    radix_scatter_kernel<Items, false, kItems, 8><<<nblocks, kThreads, 0, s>>>(
+        it, kout, vout, n_ptr, cap, shift, bits, hist, nb, totals, ex);
+}
+
 int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid &gr, int T,
                    void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,
                    int32_t *tile_range, hipStream_t s) {
@@ -2322,9 +2350,9 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     exA.wtot = totA + DA;
     exA.e_own = ownE;
     exA.e_own_cap = (uint32_t)own_e_cap((uint64_t)m);
-    radix_scatter_kernel<<<nb, kThreads, 0, s>>>(
-        GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
-        ib.keyA, ib.valA, g.scalars + 2, mu, 0, bitsA, histA, nb, totA, exA);
+    gen_scatter(bitsA, nb, s,
+                GenItems<RowStream, true>{rs, g.scalars, (uint32_t)n, gr, first, g.scalars + 3},
+                ib.keyA, ib.valA, g.scalars + 2, mu, 0, histA, nb, totA, exA);
     // pass B: each row's instances, stably by column, straight to ids_out
     const SegStream ss{ib.valA, ib.keyB, ib.keyA,     totA,          g.scalars,
                        gr.gx,   gr.gy,   tile_range, g.scalars + 7, ownE};
@@ -2338,9 +2366,9 @@ int sort_emit_impl(const float *uv, const int32_t *radius, int64_t n, const Grid
     ex.row_blocks = g.scalars + 7;
     ex.tile_range = tile_range;
     ex.gx = gr.gx;
-    radix_scatter_kernel<<<nbB, kThreads, 0, s>>>(
-        GenItems<SegStream, false, true>{ss, g.scalars + 2, mu, gr, firstB, nullptr}, nullptr,
-        ids_out, g.scalars + 5, mu, 24, bitsB, histB, nbB, totalsB, ex);
+    gen_scatter(bitsB, nbB, s,
+                GenItems<SegStream, false, true>{ss, g.scalars + 2, mu, gr, firstB, nullptr},
+                nullptr, ids_out, g.scalars + 5, mu, 24, histB, nbB, totalsB, ex);
     return gsr_launch_status("sort_emit");
   }
   const int nbits = tile_bits(T);

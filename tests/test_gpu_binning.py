@@ -66,6 +66,7 @@ CASES = [(200_000, 1024, 1024, 1, False, 0),   # 64 x 64 = 4096 tiles
          (20_000, 1024, 1024, 6, False, 300),  # long single-splat runs
          (5_000, 640, 360, 7, True, 64),
          (30_000, 4200, 700, 12, False, 0),    # 263 x 44 tiles: wider than row binning
+         (60_000, 3000, 2200, 15, True, 0),    # 188 x 138 tiles: 8-bit rows and columns
          (2_000, 16, 700, 13, True, 0),        # one tile wide
          (2_000, 700, 16, 14, True, 0)]        # one tile high
 
