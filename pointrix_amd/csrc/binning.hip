@@ -1662,7 +1662,27 @@ __global__ __launch_bounds__(kThreads) void gen_hist_kernel(
       // this lane's splat: its instances inside [q0, q1), relative to its
       // first instance, row by row of its rectangle
       const uint32_t a = max(cur.off, q0), b = min(cur.end, q1);
-      if (cur.off != 0xFFFFFFFFu && a < b) {
+      if constexpr (Stream::kOneRow) {
+        // one row (segments, or a splat's rows): items a .. b - 1 are the
+        // digits d0 .. d1 - 1 with d1 <= the grid side <= D (no full cycle,
+        // no wrap; a run of all D digits ends at the extra slot s_d[D])
+        if (cur.off != 0xFFFFFFFFu && a < b) {
+          const uint32_t d0 = (cur.xy & 0xFFFFu) + (a - cur.off), d1 = d0 + (b - a);
+          atomicAdd(&s_d[d0], 1);
+          atomicAdd(&s_d[d1], -1);
+          if constexpr (kW) {
+            const int32_t wt = (int32_t)((uint32_t)cur.toff >> 16);
+            atomicAdd(&s_dw[d0], wt);
+            atomicAdd(&s_dw[d1], -wt);
+            const uint32_t qf = s_rng.y;
+            if (qf >= a && qf < b) {
+              const uint32_t dq = d0 + (qf - a);
+              atomicAdd(&s_dw[dq], (int32_t)s_rng.z - wt);
+              atomicAdd(&s_dw[dq + 1], wt - (int32_t)s_rng.z);
+            }
+          }
+        }
+      } else if (cur.off != 0xFFFFFFFFu && a < b) {
         const int rw = (int)(cur.wgx & 0xFFFFu), gx = (int)(cur.wgx >> 16);
         const int x0 = (int)(cur.xy & 0xFFFFu), y0 = (int)(cur.xy >> 16);
         const int k0 = (int)(a - cur.off), k1 = (int)(b - cur.off) - 1;
