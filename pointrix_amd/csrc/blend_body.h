@@ -312,26 +312,20 @@ __device__ __forceinline__ void blend_bwd_tile2(
   const int by0 = ty * TH, byl = min(ty * TH + TH, tv.H) - 1;
   const int my_q = bfly_index<QP>(lane);
   const bool writer = (lane & (64 / QP - 1)) == 0 && my_q < Q;
-  // destination of this lane's quantity for splat id: my_dst[id * my_stride +
-  // my_off].  (du, dv) and dopacity go to their own dense arrays when given
-  // (then the record rows are compact, [da, db, dc, dfeature...]), else to
-  // the full row
-  float *my_dst = d_splat;
-  int my_stride = ld;
-  int64_t my_off;
-  {
+
+  // per batch slot, the tile's gradient record summed over its strips (LDS
+  // float atomics from each strip's butterfly), then one global atomic per
+  // non-zero quantity for the whole tile instead of one per strip.
+  // Destination of quantity q of splat g: (du, dv) and dopacity go to their
+  // own dense arrays when given (then the record rows are compact, [da, db,
+  // dc, dfeature...]), else to the full row
+  __shared__ float s_acc[BD * Q];
+  auto grad_addr = [&](int q, int g) -> float * {
+    if (q < 2 && d_uv_sep != nullptr) return d_uv_sep + 2 * (int64_t)g + q;
+    if (q == 5 && d_op_sep != nullptr) return d_op_sep + ((int64_t)g - tv.op_offset);
     const bool compact = d_uv_sep != nullptr && d_op_sep != nullptr;
-    my_off = compact ? my_q - (my_q < 5 ? 2 : 3) : my_q;
-    if (my_q < 2 && d_uv_sep != nullptr) {
-      my_dst = d_uv_sep;
-      my_stride = 2;
-      my_off = my_q;
-    } else if (my_q == 5 && d_op_sep != nullptr) {
-      my_dst = d_op_sep;
-      my_stride = 1;
-      my_off = -tv.op_offset;
-    }
-  }
+    return d_splat + (int64_t)g * ld + (compact ? q - (q < 5 ? 2 : 3) : q);
+  };
 
   PixBwd<C> pp[PP];
   int tile_last = 0;
@@ -375,6 +369,7 @@ __device__ __forceinline__ void blend_bwd_tile2(
   for (int bend = stop; bend > start; bend -= BD) {
     const int bstart = max(start, bend - BD);
     __syncthreads();
+    for (int i = (int)threadIdx.x; i < BD * Q; i += NT) s_acc[i] = 0.0f;
     const int j = bstart + (int)threadIdx.x;
     unsigned mask = 0u;
     if ((int)threadIdx.x < BD && j < bend) {
@@ -445,7 +440,18 @@ __device__ __forceinline__ void blend_bwd_tile2(
         contrib = pix_bwd_step<C, QP, BG>(pp[h], sp, pn[h], sm.f + k, BD, v) | contrib;
       if (__ballot(contrib) == 0ull) continue;  // wave-uniform
       Bfly<QP, 32>::run(v, lane);
-      if (writer) atomicAdd(my_dst + ((int64_t)sm.id[k] * (int64_t)my_stride + my_off), v[0]);
+      if (writer) atomicAdd(&s_acc[k * Q + my_q], v[0]);
+    }
+    __syncthreads();
+    // the batch's records to memory: lanes over (slot, quantity), adding 0
+    // would change nothing
+    const int nq = (bend - bstart) * Q;
+    for (int i = (int)threadIdx.x; i < nq; i += NT) {
+      const float val = s_acc[i];
+      if (val != 0.0f) {
+        const int k = i / Q;
+        atomicAdd(grad_addr(i - k * Q, sm.id[k]), val);
+      }
     }
   }
 }
