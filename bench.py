@@ -36,16 +36,16 @@ HBM_PEAK_GBS = 8000.0
 # full3m workload, each scaled by the counter's reading for a copy of known
 # size; summary and commands in PMC_SOURCE
 PMC_SOURCE = "profiles/blend_full3m_pmc.md"
-PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 65176 * 1024 * 2 + 84624 * 1024,
-                     "alpha_blending_fwd": 22061 * 1024 * 2 + 51523 * 1024,
-                     "preprocess_bwd": 296240 * 1024 * 2 + 680019 * 1024,
-                     "preprocess_fwd": 255527 * 1024 * 2 + 175784 * 1024}
+PMC_TRAFFIC_BYTES = {"alpha_blending_bwd": 64996 * 1024 * 2 + 46788 * 1024,
+                     "alpha_blending_fwd": 22057 * 1024 * 2 + 51523 * 1024,
+                     "preprocess_bwd": 296245 * 1024 * 2 + 680019 * 1024,
+                     "preprocess_fwd": 255529 * 1024 * 2 + 175784 * 1024}
 # the blend kernels' instruction counters (rocprofv3 --pmc SQ_*) on the
 # full3m workload, from the same summary: VALU instructions per dispatch and
 # their ratio to the dispatch's CU-cycles (256 CUs x GRBM_GUI_ACTIVE / 8 XCDs),
 # against a bound of one wave64 VALU instruction per CU-cycle
-VALU_PMC = {"alpha_blending_bwd": {"pmc_valu_insts": 1.422e8, "pmc_valu_issue_frac": 0.71},
-            "alpha_blending_fwd": {"pmc_valu_insts": 8.298e7, "pmc_valu_issue_frac": 1.08}}
+VALU_PMC = {"alpha_blending_bwd": {"pmc_valu_insts": 1.449e8, "pmc_valu_issue_frac": 0.85},
+            "alpha_blending_fwd": {"pmc_valu_insts": 8.298e7, "pmc_valu_issue_frac": 1.09}}
 
 
 def rotmat_to_quat(R):

@@ -201,6 +201,15 @@ int gsr_sort_count_async(const float *uv, const float *depth, const int32_t *rad
                          const float *conic, const float *opacity, int64_t n, int W, int H,
                          int tile_w, int tile_h, void *ws_gauss, int32_t *m_out,
                          int32_t *m_host, void *event, void *stream);
+/* As gsr_sort_count_tight, and *m_high (device int32) receives max(*m_high,
+ * M): the largest count since the caller last reset it, stored by the kernel
+ * that sums M (no launch of its own).  For a caller that replays a captured
+ * sort with a fixed instance capacity and checks once, after many replays,
+ * whether any of them overflowed it.  No msplat counterpart. */
+int gsr_sort_count_high(const float *uv, const float *depth, const int32_t *radius,
+                        const float *conic, const float *opacity, int64_t n, int W, int H,
+                        int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, int32_t *m_high,
+                        void *stream);
 int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *conic,
                         const float *opacity, int64_t n, int W, int H, int tile_w, int tile_h,
                         void *ws_gauss, int64_t m, void *ws_inst, int32_t *ids_out,

@@ -49,6 +49,7 @@ SIGNATURES = {
     "gsr_mv_preprocess_bwd": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _P, _P, _F, _F, _I, _P,
                                    _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "gsr_sort_count_tight": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P]),
+    "gsr_sort_count_high": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P, _P]),
     "gsr_sort_count_async": (_I, [_P, _P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _P, _P, _P,
                                   _P]),
     "gsr_sort_emit_tight": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _I, _P, _I64, _P, _P, _P,

@@ -183,6 +183,7 @@ struct TotCompact {
   uint32_t naux;
   uint32_t *aux_out;
   uint32_t *host_out = nullptr;
+  uint32_t *high = nullptr;  // receives max(*high, the sum)
   __device__ void operator()(uint32_t t) const {
     uint32_t a = 0;
 #pragma unroll 4
@@ -193,6 +194,7 @@ struct TotCompact {
       *out = t;
       *aux_out = sum;
       if (host_out != nullptr) *host_out = sum;
+      if (high != nullptr) *high = max(*high, sum);
     }
   }
 };
@@ -2229,7 +2231,7 @@ int scratch_of_current_device(char **out) {
 
 int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, int64_t n,
                     const Grid &gr, void *ws_gauss, int32_t *m_out, hipStream_t s,
-                    int32_t *m_host = nullptr, void *event = nullptr) {
+                    int32_t *m_host = nullptr, void *event = nullptr, int32_t *m_high = nullptr) {
   GaussWs g;
   carve_gauss(ws_gauss, (uint64_t)n, g);
   const uint32_t cap = (uint32_t)(n > 0 ? n : 1);
@@ -2252,14 +2254,15 @@ int sort_count_impl(const float *uv, const float *depth, const int32_t *radius, 
         LoadRectFlag{g.rects},
         StoreCompact{depth, g.sb.keyA, g.sb.valA, g.rects, reinterpret_cast<uint2 *>(g.masks)},
         nullptr, (uint32_t)n, g.sb.partials, nb,
-        TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out, m_host_dev});
+        TotCompact{g.scalars, g.bsum, nb, (uint32_t *)m_out, m_host_dev, (uint32_t *)m_high});
   } else {
     const uint32_t ncb = (uint32_t)gsr_blocks(n, kThreads);
     count_tight_kernel<<<ncb, kThreads, 0, s>>>(uv, radius, (uint32_t)n, gr, g.cnt, g.masks,
                                                 g.bsum);
     run_scan(LoadFlag{g.cnt}, StoreCompact{depth, g.sb.keyA, g.sb.valA, nullptr, nullptr},
              nullptr, (uint32_t)n, g.sb.partials,
-             TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out, m_host_dev}, s);
+             TotCompact{g.scalars, g.bsum, ncb, (uint32_t *)m_out, m_host_dev, (uint32_t *)m_high},
+             s);
   }
   {
     const int rc = copy_count_to_host(m_out, m_host, event, s, m_host_dev != nullptr);
@@ -2517,6 +2520,21 @@ int gsr_sort_emit_tight(const float *uv, const int32_t *radius, const float *con
   const Grid gr{gx, gy, tile_w, tile_h, nullptr, 0, W, H, conic, opacity};
   return sort_emit_impl(uv, radius, n, gr, gx * gy, ws_gauss, m, ws_inst, ids_out, tile_range,
                         (hipStream_t)stream);
+}
+
+int gsr_sort_count_high(const float *uv, const float *depth, const int32_t *radius,
+                        const float *conic, const float *opacity, int64_t n, int W, int H,
+                        int tile_w, int tile_h, void *ws_gauss, int32_t *m_out, int32_t *m_high,
+                        void *stream) {
+  GSR_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && W > 0 && H > 0 && tile_w > 0 && tile_h > 0 &&
+                  (W + tile_w - 1) / tile_w < 65536 && (H + tile_h - 1) / tile_h < 65536,
+              "sort_count: bad sizes");
+  GSR_REQUIRE((conic == nullptr) == (opacity == nullptr), "sort_count: conic without opacity");
+  GSR_REQUIRE(m_high != nullptr, "sort_count_high: no high-water word");
+  const Grid gr{(W + tile_w - 1) / tile_w, (H + tile_h - 1) / tile_h, tile_w, tile_h, nullptr, 0,
+                W, H, conic, opacity};
+  return sort_count_impl(uv, depth, radius, n, gr, ws_gauss, m_out, (hipStream_t)stream, nullptr,
+                         nullptr, m_high);
 }
 
 int gsr_sort_count(const float *uv, const float *depth, const int32_t *radius, int64_t n,

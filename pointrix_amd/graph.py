@@ -15,7 +15,10 @@ at the new size and replayed, so every result the caller sees is complete.
 The object that sizes the buffers (the "sizer": a renderer's sort workspace,
 or the batched renderer) exposes ``capacity`` (None = exact, with a host
 read), ``instances()`` (host read of the device count of the last forward)
-and ``count_tensor()`` (that device count, an int32[1] tensor).
+and ``count_tensor()`` (that device count, an int32[1] tensor).  A sizer may
+also expose ``high_water_tensor()``: the largest count of its fixed-capacity
+sorts since it was last zeroed, kept by the library's count kernel, which
+then replaces the step's own tracking kernel.
 """
 from __future__ import annotations
 
@@ -36,7 +39,7 @@ class CapturedStep:
     replace ``.grad``) never leave the graph writing into freed memory.
 
     The graph also keeps the largest instance count it has seen on the device
-    (one small kernel).  ``replay(check=True)`` reads it after the replay (a
+    (the sizer's high-water word when it has one, else one small kernel).  ``replay(check=True)`` reads it after the replay (a
     host wait) and re-captures when it outgrew the capacity;
     ``replay(check=False)`` does not wait, and ``overflowed()`` tells later
     whether any unchecked replay since the last check overflowed (its
@@ -50,25 +53,34 @@ class CapturedStep:
         self.grads = {}
         self.captures = 0
         self.max_count = None
+        self.own_max = True  # False: the sizer's high-water word is read instead
 
     def _step(self):
         for p in self.leaves.values():
             p.grad = None
         loss = self.step_fn()
-        m = self.sizer.count_tensor()
-        if m is not None:
-            torch.maximum(self.max_count, m, out=self.max_count)
+        if self.own_max:
+            m = self.sizer.count_tensor()
+            if m is not None:
+                torch.maximum(self.max_count, m, out=self.max_count)
         return loss
+
+    def _high(self):
+        hw = getattr(self.sizer, "high_water_tensor", None)
+        return hw() if hw is not None else None
 
     def capture(self):
         sz = self.sizer
         sz.capacity = None
         dev = next(iter(self.leaves.values())).device
         self.max_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.own_max = True
         self._step()                                   # eager: exact instance count
         torch.cuda.synchronize()
         m = sz.instances()
         sz.capacity = max(self.min_capacity, int(math.ceil(m * self.headroom)))
+        self._step()                                   # fixed capacity: creates the high-water word
+        self.own_max = self._high() is None
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
@@ -79,16 +91,22 @@ class CapturedStep:
         with torch.cuda.graph(self.graph):
             self.loss = self._step()
         self.grads = {k: p.grad for k, p in self.leaves.items()}
-        self.max_count.zero_()
+        self._reset_max()
         self.captures += 1
         return self
+
+    def _max_tensor(self):
+        return self.max_count if self.own_max else self._high()
+
+    def _reset_max(self):
+        self._max_tensor().zero_()
 
     def overflowed(self) -> bool:
         """Host read of the largest instance count seen since the last check:
         True (and the step captured again at the new size) when it exceeded
         the capacity."""
-        m = int(self.max_count.item())
-        self.max_count.zero_()
+        m = int(self._max_tensor().item())
+        self._reset_max()
         if m > self.sizer.capacity:
             self.capture()
             return True

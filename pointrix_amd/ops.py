@@ -232,12 +232,15 @@ class SortWorkspace:
     of the device instance count per sort.  An integer capacity skips that
     read (as a captured graph requires): the instance list is allocated at the
     capacity, emission never writes past it, and ``instances()`` returns the
-    true count of the last sort so the caller can detect an overflow."""
+    true count of the last sort so the caller can detect an overflow;
+    ``high_water_tensor()`` holds the largest count of the fixed-capacity
+    sorts since it was last zeroed (the library's count kernel updates it)."""
 
     def __init__(self):
         self.gauss = None
         self.inst = None
         self.m_dev = None
+        self.high_dev = None
         self.capacity = None
 
     def gauss_buf(self, n, device):
@@ -256,6 +259,11 @@ class SortWorkspace:
         if self.m_dev is None or self.m_dev.device != device:
             self.m_dev = torch.zeros(1, dtype=torch.int32, device=device)
         return self.m_dev
+
+    def high_buf(self, device):
+        if self.high_dev is None or self.high_dev.device != device:
+            self.high_dev = torch.zeros(1, dtype=torch.int32, device=device)
+        return self.high_dev
 
     def host_count(self, device):
         """Pinned host int32[1] the instance count reaches ahead of the depth
@@ -293,6 +301,11 @@ class SortWorkspace:
     def count_tensor(self):
         """The device int32[1] the instance count of each sort is written to."""
         return self.m_dev
+
+    def high_water_tensor(self):
+        """The device int32[1] holding the largest instance count of the
+        fixed-capacity sorts since it was last zeroed (None before the first)."""
+        return self.high_dev
 
 
 def set_binning_mode(mode: int) -> int:
@@ -358,8 +371,8 @@ def sort_gaussian(uv, depth, W, H, radius, tiles, tile_w=TILE, tile_h=TILE, work
         last_instance_count = m
     else:
         with _span("sort_count", dev):
-            call("gsr_sort_count_tight", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H),
-                 tile_w, tile_h, ptr(g), ptr(m_dev), st)
+            call("gsr_sort_count_high", ptr(u), ptr(d), ptr(r), tc, to, n, int(W), int(H),
+                 tile_w, tile_h, ptr(g), ptr(m_dev), ptr(ws.high_buf(dev)), st)
         m = int(ws.capacity)
     ids = torch.empty(m, dtype=torch.int32, device=dev)
     tile_range = torch.empty(gx * gy, 2, dtype=torch.int32, device=dev)

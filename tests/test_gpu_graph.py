@@ -5,7 +5,8 @@
   replays;
 * a capacity below the instance count is detected and the step re-captured,
   so the replayed result is still complete;
-* a fixed-capacity sort returns the same ids and tile ranges as the exact one.
+* a fixed-capacity sort returns the same ids and tile ranges as the exact one,
+  and its high-water word keeps the largest instance count.
 """
 import pytest
 import torch
@@ -110,3 +111,33 @@ def test_fixed_capacity_sort_equals_exact(device):
     assert torch.equal(capped["ids"][:m], exact["ids"])
     r.workspace.capacity = None
     r.keep_intermediates = False
+
+
+def test_high_water_keeps_largest_count(device):
+    """Fixed-capacity sorts (gsr_sort_count_high): the workspace's high-water
+    word holds the largest instance count since it was zeroed, stored by the
+    count kernel -- a later, smaller count leaves it, zeroing restarts it."""
+    ws = ops.SortWorkspace()
+    cases = []
+    for seed, n, r_max in ((71, 4000, 30), (72, 4000, 60), (73, 4000, 10)):
+        g = torch.Generator().manual_seed(seed)
+        uv = (torch.rand(n, 2, generator=g) * torch.tensor([320.0, 200.0])).to(device)
+        depth = (torch.rand(n, 1, generator=g) * 10 + 0.5).to(device)
+        radius = torch.randint(0, r_max, (n,), generator=g, dtype=torch.int32).to(device)
+        ws.capacity = None
+        ops.sort_gaussian(uv, depth, 320, 200, radius, None, workspace=ws)
+        cases.append((uv, depth, radius, ws.instances()))
+    counts = [c[3] for c in cases]
+    assert counts[1] > counts[0] > counts[2] > 0
+    ws.capacity = max(counts) + 100
+    seen = []
+    for uv, depth, radius, m in cases:
+        ops.sort_gaussian(uv, depth, 320, 200, radius, None, workspace=ws)
+        assert ws.instances() == m
+        seen.append(int(ws.high_water_tensor().item()))
+    assert seen == [counts[0], counts[1], counts[1]]
+    ws.high_water_tensor().zero_()
+    uv, depth, radius, m = cases[2]
+    ops.sort_gaussian(uv, depth, 320, 200, radius, None, workspace=ws)
+    assert int(ws.high_water_tensor().item()) == counts[2]
+    ws.capacity = None
